@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark: exact N-Queens count on one MI355X, DFS nodes/s.
+
+Workload (default): N=19, preset 5 -- all 100,706 canonical constellations
+counted exactly in one batched run (work items split at depth `--split`,
+one persistent launch pulling them from a device queue).  A "step" is one
+complete exact count of N=19 with the task set already resident in HBM:
+expansion, persistent DFS and the weighted reduction; every step's total is
+checked against A000170.  The plain one-lane-per-constellation kernel is
+timed on the same records beside it, and the CPU oracle (a C/OpenMP
+restatement of the reference's dfs_iter) on a bounded sample of them.
+
+One node = one candidate extraction of the reference dfs_iter
+(13Bit_codon/18Py_constellations_GPU_cuda_codon.py:368-562); node counts come
+from a node-counting run of the same kernel (checked against the oracle by
+the parity tests).  Roofline: 12 algorithmic int32 ops per node against the
+INT32 VALU peak of 256 CUs x 64 lanes x 2.4 GHz.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "n-queens_amd", ROOT):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = "N=19/21 exact-count wall time & DFS nodes/s: batched+cost-model vs plain kernel"
+OPS_PER_NODE = 12
+PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12   # 39.32 T int32 VALU ops/s
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=19)
+    ap.add_argument("--preset", type=int, default=5)
+    ap.add_argument("--split", type=int, default=None)
+    ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--lds", type=int, default=None)
+    ap.add_argument("--chunk", type=int, default=None)
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="fifo")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration (0 = skip)")
+    ap.add_argument("--json-out", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the count shards by constellation; see DESIGN.md)"}))
+        return 2
+
+    import nqueens
+    from nqueens import autotune
+    known = json.loads((ROOT / "tests" / "golden" / "known.json").read_text())
+    N, preset = args.n, args.preset
+    expect_total, expect_unique = known["readme_total_unique"][str(N)]
+
+    ctx = nqueens.Context(0)
+    info = ctx.device_info()
+    log(f"[bench] device {info}")
+    t0 = time.perf_counter()
+    rec = nqueens.gen_constellations(N, preset)
+    gen_s = time.perf_counter() - t0
+    log(f"[bench] N={N} preset={preset} constellations={rec.shape[0]} gen {gen_s:.2f}s")
+
+    sched = autotune.best_sched(N, preset)
+    for k, v in (("split_depth", args.split), ("block", args.block), ("lds_stack", args.lds), ("chunk", args.chunk)):
+        if v is not None:
+            setattr(sched, k, v)
+    sched.use_graph = args.graph
+    order = None
+    mlp_info = None
+    if args.scheduler == "mlp":
+        from nqueens import costmodel
+        model = costmodel.default_model()
+        t0 = time.perf_counter()
+        order = model.order(N, rec, ctx=ctx)
+        mlp_info = {"order_s": round(time.perf_counter() - t0, 4), "trained_on": model.trained_on}
+    ctx.load(N, rec, order)
+
+    # node count of the workload (untimed, same kernel with the counter on)
+    total, _ = ctx.run_batched(sched, count_nodes=True)
+    _, nodes = ctx.results(weighted=False, nodes=True)
+    nodes_total = int(nodes.sum())
+    assert total == expect_total, (total, expect_total)
+    log(f"[bench] nodes={nodes_total} sched={dict((f, getattr(sched, f)) for f, _ in sched._fields_)}")
+
+    for _ in range(args.warmup):
+        tot, _ = ctx.run_batched(sched)
+        assert tot == expect_total
+    kernel_ms, run_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tot, t = ctx.run_batched(sched)
+        if tot != expect_total:
+            raise SystemExit(f"wrong total {tot} != {expect_total}")
+        kernel_ms.append(t.kernel_ms)
+        run_ms.append(t.total_ms)
+        items = int(t.items)
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    value = nodes_total / (wall / args.steps)
+    f90, f180 = ctx.symmetric_counts(N)
+    unique = nqueens.unique_count(expect_total, f90, f180)
+    assert unique == expect_unique
+    log(f"[bench] batched {ms_per_step:.2f} ms/count, {value / 1e9:.2f} G nodes/s, kernel {np.mean(kernel_ms):.2f} ms")
+
+    # plain kernel on the same records (record order)
+    ctx.load(N, rec)
+    tot, _ = ctx.run_plain()
+    assert tot == expect_total
+    pk = []
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        tot, t = ctx.run_plain()
+        assert tot == expect_total
+        pk.append(t.kernel_ms)
+    plain_wall = (time.perf_counter() - t0) / max(1, args.plain_steps)
+    plain_value = nodes_total / plain_wall
+    log(f"[bench] plain {1000 * plain_wall:.2f} ms/count, {plain_value / 1e9:.2f} G nodes/s")
+
+    kavg = float(np.mean(kernel_ms))
+    achieved = nodes_total * OPS_PER_NODE / (kavg / 1000.0) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(PEAK_INT32_TOPS, 2),
+                "unit": "TOP/s", "frac": round(achieved / PEAK_INT32_TOPS, 4), "traffic": None,
+                "kernel": "k_batched", "kernel_ms": round(kavg, 3),
+                "ops_per_node": OPS_PER_NODE, "nodes_per_launch": nodes_total}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(N, rec, nodes, args.cpu_seconds)
+        log(f"[bench] cpu {cpu}")
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"exact count N={N}, preset {preset}: {rec.shape[0]} constellations, "
+                               f"batched persistent kernel, {args.scheduler} order",
+                   "N": N, "preset": preset, "constellations": int(rec.shape[0]), "work_items": items,
+                   "split_depth": sched.split_depth, "block": sched.block, "lds_stack": sched.lds_stack,
+                   "chunk": sched.chunk, "graph": sched.use_graph, "scheduler": args.scheduler,
+                   "parallelism": "single GPU"},
+        "total": expect_total, "unique": unique, "nodes": nodes_total,
+        "wall_s_per_count": round(ms_per_step / 1000, 5), "run_ms_device": round(float(np.mean(run_ms)), 3),
+        "plain": {"nodes_per_s": round(plain_value, 1), "ms_per_count": round(1000 * plain_wall, 3),
+                  "kernel_ms": round(float(np.mean(pk)), 3)},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "roofline": roofline, "cpu_baseline": cpu, "cost_model": mlp_info,
+        "device": info,
+    }
+    line = json.dumps(out)
+    print(line, flush=True)
+    if args.json_out:
+        Path(args.json_out).write_text(line + "\n")
+    ctx.close()
+    return 0
+
+
+def cpu_baseline(N: int, rec: np.ndarray, nodes: np.ndarray, seconds: float) -> dict:
+    """Oracle dfs_iter (C, OpenMP dynamic,64) on every k-th record, k chosen
+    from the GPU node counts so that the sample is about `seconds` of work."""
+    from oracle import oracle
+    cores = len(os.sched_getaffinity(0))
+    # calibrate on a small stride sample
+    probe = rec[::max(1, rec.shape[0] // 512)]
+    t0 = time.perf_counter()
+    _, _, pn = oracle.count_records(N, probe, threads=cores, schedule=1)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    rate = float(pn.sum()) / dt
+    want_nodes = rate * seconds
+    stride = max(1, int(np.ceil(float(nodes.sum()) / max(want_nodes, 1.0))))
+    sample = np.ascontiguousarray(rec[::stride])
+    t0 = time.perf_counter()
+    w, _, sn = oracle.count_records(N, sample, threads=cores, schedule=1)
+    dt = time.perf_counter() - t0
+    if not np.array_equal(sn, nodes[::stride]):
+        raise SystemExit("cpu baseline: node counts disagree with the GPU")
+    return {"value": round(float(sn.sum()) / dt, 1), "unit": "nodes/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th of the {rec.shape[0]} N={N} constellations "
+                      f"({sample.shape[0]} records, {int(sn.sum())} nodes, {dt:.1f} s), OpenMP dynamic,64",
+            "seconds": round(dt, 2)}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

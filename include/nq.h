@@ -1,0 +1,138 @@
+/*
+ * nq.h -- C ABI of libnq, the MI355X (gfx950) constellation counter.
+ *
+ * Drop-in for the constellation solver's counting step:
+ *   exec_solutions(N, constellations, use_gpu)   13Bit_codon/18Py_constellations_GPU_cuda_codon.py:1057-1275
+ *   kernel_dfs_iter_gpu launch (Codon @gpu.kernel, gpu.raw pointers)      18Py:1195-1215, kernel 18Py:149-365
+ * The reference binds its kernel through Codon's gpu.raw(list) + grid=/block=;
+ * this library is bound through ctypes with plain pointers and sizes.
+ *
+ * Records are the reference's 16-byte constellation record, u32 little
+ * endian {ld, rd, col, startijkl} (writer 18Py:2009-2016).  Per-record
+ * outputs are the weighted count (count * w, w in {2,4,8}, 18Py:1266) and,
+ * optionally, the DFS node count (one node = one extraction
+ * bit = avail & -avail of the reference dfs_iter, 18Py:368-562).
+ *
+ * All pointers passed in are caller-owned HOST memory unless the name says
+ * "dev"; nothing is retained after a call returns.  A context is bound to
+ * one device and one HIP stream and is not thread-safe.  Every entry point
+ * returns 0 or a negative nq_status; it never returns a partial count
+ * (the reference's stack "safety valve" at 18Py:354-356 returns a partial
+ * total silently -- here that case is NQ_ECAPACITY).
+ */
+#ifndef NQ_H
+#define NQ_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NQ_ABI_VERSION 1
+
+enum nq_status {
+    NQ_OK = 0,
+    NQ_EINVAL = -1,        /* bad argument (N out of 6..21, NULL pointer, size) */
+    NQ_EUNSUPPORTED = -2,  /* record outside the u32 path: P5 target (fids 8-11),
+                              shift out of range, no terminal row */
+    NQ_ECAPACITY = -3,     /* DFS stack / work-item buffer capacity exceeded */
+    NQ_EHIP = -4,          /* HIP runtime error (message in nq_last_error) */
+    NQ_ENOMEM = -5,
+    NQ_ESTATE = -6         /* call order: e.g. run before load */
+};
+
+typedef struct nq_ctx nq_ctx;
+
+/* Batched-kernel schedule (no reference counterpart: the reference launches
+ * one thread per constellation, 18Py:1195-1215, or a grid-stride batch,
+ * 14Bit_codon_GPU/359Py_pushbranch_partial.py:1781-1867). */
+typedef struct {
+    int32_t split_depth;   /* frames at this DFS depth become work items (0 = no split) */
+    int32_t block;         /* workgroup size: 64, 128 or 256 */
+    int32_t lds_stack;     /* 1: DFS stack in LDS, 0: in private (scratch) memory */
+    int32_t use_graph;     /* 1: replay the launch sequence as a captured hipGraph */
+    int32_t chunk;         /* work items a wave claims per queue fetch (>= 1) */
+    int32_t waves_per_eu;  /* persistent occupancy target; 0 = from the occupancy API */
+} nq_sched;
+
+typedef struct {
+    double kernel_ms;      /* device time of the DFS kernel(s), HIP events on the context stream */
+    double total_ms;       /* device time of the whole run (expand + DFS + finalize) */
+    uint64_t items;        /* work items after splitting */
+    uint64_t launches;     /* DFS kernel launches in the run */
+} nq_timing;
+
+/* ---- library / context -------------------------------------------------- */
+int nq_abi_version(void);
+const char *nq_last_error(void);
+int nq_ctx_create(int device, nq_ctx **out);
+int nq_ctx_destroy(nq_ctx *ctx);
+/* device properties the scheduler uses: {compute units, max LDS bytes per
+ * workgroup, wavefront size, clock kHz} */
+int nq_device_info(nq_ctx *ctx, int32_t *out4);
+
+/* ---- resident task set ---------------------------------------------------
+ * exec_solutions' input list (18Py:1057): copies m records to HBM and decodes
+ * them there (build_soa_for_range, 18Py:698-1054, and the func_meta schedule,
+ * 18Py:1103-1156) into per-record DFS descriptors.  `order` (may be NULL)
+ * is the processing order for the batched kernel (a permutation of 0..m-1);
+ * results are always reported in record order. */
+int nq_ctx_load(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const uint32_t *order);
+
+/* Plain kernel: one lane per constellation, iterative bitboard DFS with the
+ * stack in LDS, u64 wave reduction of the weighted counts.  Replaces
+ * kernel_dfs_iter_gpu (18Py:149-365) and its launch loop (18Py:1195-1215). */
+int nq_ctx_run_plain(nq_ctx *ctx, int count_nodes, uint64_t *total_out, nq_timing *t);
+
+/* Batched kernel: heavy subtrees split at sched->split_depth into work items,
+ * one persistent launch pulling items from a device queue. */
+int nq_ctx_run_batched(nq_ctx *ctx, const nq_sched *sched, int count_nodes,
+                       uint64_t *total_out, nq_timing *t);
+
+/* Per-record results of the last run (weighted count; node count when the
+ * run counted nodes, else zeros).  Either pointer may be NULL. */
+int nq_ctx_results(nq_ctx *ctx, uint64_t *weighted_out, uint64_t *nodes_out);
+
+/* One-shot form of exec_solutions(N, list, use_gpu=True) (18Py:1057-1275):
+ * load + run + results.  sched == NULL selects the plain kernel. */
+int nq_count(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const nq_sched *sched,
+             uint64_t *weighted_out, uint64_t *nodes_out, uint64_t *total_out);
+
+/* ---- Unique ---------------------------------------------------------------
+ * The constellation solvers print Unique = 0 (18Py:2160).  Unique is the
+ * orbit count of 13Bit_codon/07Py_bit_symmetry_codon.py:410-411
+ * (unique = c2+c4+c8).  This returns the number of solutions fixed by the
+ * quarter turn (F90) and by the half turn (F180); the host forms
+ * unique = (total + F180 + 2*F90) / 8. */
+int nq_symmetric_counts(nq_ctx *ctx, int N, uint64_t *f90_out, uint64_t *f180_out);
+
+/* ---- cost model (no reference counterpart; the reference orders work by a
+ * hand-made key, 14Bit_codon_GPU/359Py_pushbranch_partial.py:4342-4404) ----
+ * 3-layer FP32 MLP 24 -> 64 -> 64 -> 1 (ReLU), parameters packed as
+ * W1[64x24] b1[64] W2[64x64] b2[64] W3[64] b3[1] = NQ_MLP_NPARAM floats. */
+#define NQ_MLP_IN 24
+#define NQ_MLP_H 64
+#define NQ_MLP_NPARAM (64 * 24 + 64 + 64 * 64 + 64 + 64 + 1)
+
+/* Features of the resident task set (row-major m x 24 floats, record order). */
+int nq_ctx_features(nq_ctx *ctx, float *feat_out);
+/* Forward pass: x[m x 24] -> y[m]. */
+int nq_mlp_forward(nq_ctx *ctx, const float *params, const float *x, uint64_t m, float *y_out);
+/* Mean-squared-error loss and its gradient w.r.t. the packed parameters. */
+int nq_mlp_loss_grad(nq_ctx *ctx, const float *params, const float *x, const float *y,
+                     uint64_t m, float *loss_out, float *grad_out);
+/* Adam training on device: `epochs` passes of minibatches of `batch` rows in
+ * the order `perm` (m entries per epoch, epochs*m total), learning rate lr.
+ * params is updated in place; loss_out[e] receives epoch e's mean loss. */
+int nq_mlp_train(nq_ctx *ctx, float *params, const float *x, const float *y, uint64_t m,
+                 const uint32_t *perm, int epochs, int batch, float lr, float *loss_out);
+
+/* ---- host-side descriptor check (no GPU needed) ---------------------------
+ * The same decode the device runs, executed on the host for tests: per record
+ * {fid, start, jmark, endmark, mark1, mark2, w, free, T, status}. */
+int nq_decode_host(int N, const uint32_t *rec4, uint64_t m, int32_t *out10);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NQ_H */

@@ -1,0 +1,158 @@
+// nq_dfs.h -- the per-lane bitboard DFS engine shared by the plain, expand
+// and batched kernels (nq_kernels.hip).  Host-compilable so that the engine
+// logic can be exercised off the GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "nq_kernels.h"
+
+#ifndef NQ_HD
+#define NQ_HD __host__ __device__
+#endif
+
+namespace nqk {
+
+// ---------------------------------------------------------------------------
+// per-lane schedule registers
+// ---------------------------------------------------------------------------
+struct Sched {
+    uint32_t t0, t1, t2, t3, t4;  // parameter bytes for depths 0..19
+    uint32_t T;                   // terminal depth
+    uint32_t tm;                  // terminal free mask
+};
+
+// Byte d of the 20-byte table.  v_perm_b32 picks byte (d & 7) of a register
+// pair directly, which keeps the table in VGPRs (an indexed select over the
+// five words is otherwise lowered to a scratch array).  Selector byte 0x0c
+// yields zero, so only byte 0 of each result is non-zero.
+NQ_HD inline uint32_t pbyte(const Sched &S, uint32_t d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t sel = 0x0c0c0c00u | (d & 7u);
+    const uint32_t a = __builtin_amdgcn_perm(S.t1, S.t0, sel);
+    const uint32_t b = __builtin_amdgcn_perm(S.t3, S.t2, sel);
+    const uint32_t c = __builtin_amdgcn_perm(0u, S.t4, sel);
+    return (d < 8u) ? a : ((d < 16u) ? b : c);
+#else
+    const uint32_t w[5] = {S.t0, S.t1, S.t2, S.t3, S.t4};
+    return (w[d >> 2] >> ((d & 3u) * 8u)) & 0xFFu;
+#endif
+}
+
+NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
+{
+    uint4 a = sched[2 * c], b = sched[2 * c + 1];
+    Sched S;
+    S.t0 = a.x; S.t1 = a.y; S.t2 = a.z; S.t3 = a.w; S.t4 = b.x;
+    S.T = b.y & 0xFFu;
+    S.tm = b.z;
+    return S;
+}
+
+// Solutions below a pre-terminal frame (depth T-1) without expanding it.
+// For a placement `bit` the terminal free set is F & ~(bit | bit<<s | bit>>s),
+// F = free set of the terminal row before that placement.  A bit is a dead
+// end exactly when F lies inside those (at most three) squares, so the dead
+// bits are the intersection of spread(f) over the bits f of F, empty when F
+// has four or more bits.
+NQ_HD inline uint32_t term_count(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
+                                               uint32_t p, uint32_t tm, uint32_t nsh)
+{
+    const uint32_t s = (p & 3u) + 1u;
+    const uint32_t X = (ld << s) | ((p >> 2) & 3u) | (rd >> s) | (((p >> 4) & 3u) << nsh) | col;
+    const uint32_t F = tm & ~X;
+    const uint32_t f1 = F & (0u - F);
+    const uint32_t F2 = F ^ f1;
+    const uint32_t f2 = F2 & (0u - F2);
+    const uint32_t F3 = F2 ^ f2;
+    const uint32_t f3 = F3 & (0u - F3);
+    const uint32_t F4 = F3 ^ f3;
+    uint32_t B = f1 | (f1 >> s) | (f1 << s);
+    B &= f2 ? (f2 | (f2 >> s) | (f2 << s)) : ~0u;
+    B &= f3 ? (f3 | (f3 >> s) | (f3 << s)) : ~0u;
+    B = F4 ? 0u : B;
+    B = F ? B : ~0u;
+    return __builtin_popcount(avail & ~B);
+}
+
+// ---------------------------------------------------------------------------
+// DFS stacks: LDS (entry i of lane at base + i*stride) or private memory
+// ---------------------------------------------------------------------------
+struct LdsStack {
+    uint4 *base;
+    uint32_t stride;
+    NQ_HD inline void put(int i, uint4 v) { base[i * stride] = v; }
+    NQ_HD inline uint4 get(int i) const { return base[i * stride]; }
+};
+struct PrivStack {
+    uint4 a[kStackCap];
+    NQ_HD inline void put(int i, uint4 v) { a[i] = v; }
+    NQ_HD inline uint4 get(int i) const { return a[i]; }
+};
+
+// Stack entry: {ld, rd, col, avail | depth<<19 | param<<24}; avail < 2^19.
+NQ_HD inline uint4 pack_frame(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
+                                            uint32_t d, uint32_t p)
+{
+    return make_uint4(ld, rd, col, avail | (d << 19) | (p << 24));
+}
+
+// One DFS from a frame at depth d (d <= T-2, avail != 0 after jmark) until
+// its subtree is exhausted.  SPLIT > 0: children created at depth `split`
+// are handed to emit() instead of being expanded.
+template <bool NODES, bool SPLIT, class Stack, class Emit>
+NQ_HD inline void dfs_subtree(uint32_t mask, uint32_t nsh, const Sched &S, uint32_t ld,
+                                            uint32_t rd, uint32_t col, uint32_t avail, uint32_t d,
+                                            uint32_t split, Stack &st, uint64_t &cnt, uint64_t &nodes,
+                                            Emit &&emit)
+{
+    uint32_t p = pbyte(S, d);
+    const uint32_t Tm1 = S.T - 1u;
+    int sp = 0;
+    for (;;) {
+        const uint32_t s = (p & 3u) + 1u;
+        const uint32_t bit = avail & (0u - avail);
+        avail ^= bit;
+        uint32_t nld = ((ld | bit) << s) | ((p >> 2) & 3u);
+        const uint32_t nrd = ((rd | bit) >> s) | (((p >> 4) & 3u) << nsh);
+        const uint32_t ncol = col | bit;
+        uint32_t nf = mask & ~(nld | nrd | ncol);
+        const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
+        const uint32_t cd = d + 1u;
+        const uint32_t cp = pbyte(S, cd);
+        bool ok = (nf != 0u) && (((p & 0x40u) == 0u) || (look != 0u));
+        if (cp & 0x80u) { nf &= ~1u; nld |= 1u; }
+        ok = ok && (nf != 0u);
+        if (NODES) nodes += 1u;
+        if (cd == Tm1) {
+            if (ok) {
+                cnt += term_count(nld, nrd, ncol, nf, cp, S.tm, nsh);
+                if (NODES) nodes += (uint32_t)__builtin_popcount(nf);
+            }
+            ok = false;
+        }
+        if (SPLIT && ok && cd == split) {
+            emit(nld, nrd, ncol, nf);
+            ok = false;
+        }
+        if (ok) {
+            if (avail) { st.put(sp, pack_frame(ld, rd, col, avail, d, p)); sp++; }
+            ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; p = cp;
+        } else if (avail == 0u) {
+            if (sp == 0) break;
+            sp--;
+            const uint4 e = st.get(sp);
+            ld = e.x; rd = e.y; col = e.z;
+            avail = e.w & 0x7FFFFu;
+            d = (e.w >> 19) & 31u;
+            p = e.w >> 24;
+        }
+    }
+}
+
+struct NoEmit {
+    NQ_HD inline void operator()(uint32_t, uint32_t, uint32_t, uint32_t) const {}
+};
+
+
+}  // namespace nqk

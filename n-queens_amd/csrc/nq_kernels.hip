@@ -1,0 +1,507 @@
+// nq_kernels.hip -- gfx950 kernels of libnq: record decode, the per-lane
+// bitboard DFS (plain and persistent/batched forms), work-item expansion,
+// the weighted-count reduction and the symmetric-solution enumerator.
+//
+// Work unit: one DFS node is one extraction bit = avail & -avail of the
+// reference dfs_iter (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:368-562).
+// All masks are u32 (N <= 21); counts are u64.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "nq_task.h"
+#include "nq_kernels.h"
+#include "nq_dfs.h"
+
+namespace nqk {
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Count + nodes of a whole constellation from its descriptor.
+template <bool NODES, class Stack>
+__device__ __forceinline__ void run_root(uint32_t mask, uint32_t nsh, const Sched &S, uint4 root, Stack &st,
+                                         uint64_t &cnt, uint64_t &nodes)
+{
+    cnt = 0; nodes = 0;
+    if (root.w == 0u) return;
+    if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
+    if (S.T == 1u) {
+        cnt = term_count(root.x, root.y, root.z, root.w, pbyte(S, 0), S.tm, nsh);
+        if (NODES) nodes = (uint32_t)__builtin_popcount(root.w);
+        return;
+    }
+    dfs_subtree<NODES, false>(mask, nsh, S, root.x, root.y, root.z, root.w, 0u, 0u, st, cnt, nodes, NoEmit{});
+}
+
+// ---------------------------------------------------------------------------
+// decode: records -> root frames + schedule descriptors
+// sched layout per record: uint4 {t0,t1,t2,t3}, uint4 {t4, T | wshift<<8, tm, status}
+// ---------------------------------------------------------------------------
+__global__ void k_decode(int N, const uint4 *__restrict__ rec, uint32_t m, uint4 *__restrict__ root,
+                         uint4 *__restrict__ sched, uint32_t *__restrict__ err)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    const uint4 r = rec[c];
+    nqt::Desc D;
+    nqt::decode(N, r.x, r.y, r.z, r.w, D);
+    if (D.status != nqt::kOk) atomicOr(err, 1u << D.status);
+    root[c] = make_uint4(D.ld, D.rd, D.col, D.avail);
+    sched[2 * c] = make_uint4(D.tab[0], D.tab[1], D.tab[2], D.tab[3]);
+    sched[2 * c + 1] = make_uint4(D.tab[4], D.T | (D.wshift << 8), D.tm, (uint32_t)D.status);
+}
+
+// ---------------------------------------------------------------------------
+// plain kernel: one lane per constellation, LDS stack, u64 wave reduction
+// ---------------------------------------------------------------------------
+template <bool NODES>
+__global__ void k_plain(int N, const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
+                        unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ nodes_out,
+                        unsigned long long *__restrict__ total, uint32_t stack_depth)
+{
+    extern __shared__ uint4 lds[];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
+    uint64_t wsum = 0;
+    if (c < m) {
+        const Sched S = load_sched(sched, c);
+        const uint32_t wsh = (sched[2 * c + 1].y >> 8) & 3u;
+        LdsStack st{lds + threadIdx.x, blockDim.x};
+        uint64_t cnt, nn;
+        run_root<NODES>(mask, nsh, S, root[c], st, cnt, nn);
+        wsum = cnt << wsh;
+        weighted[c] = wsum;
+        if (NODES) nodes_out[c] = nn;
+    }
+    (void)stack_depth;
+    wsum = wave_sum_u64(wsum);
+    if (lane_id() == 0 && wsum) atomicAdd(total, (unsigned long long)wsum);
+}
+
+// ---------------------------------------------------------------------------
+// expansion: frames at depth `split` become work items
+//   pass 0 counts items per position q (processing order), pass 1 writes them
+//   at off[q].. ; nodes above the split depth are counted here.
+// ---------------------------------------------------------------------------
+template <bool NODES, bool WRITE>
+__global__ void k_expand(int N, const uint4 *__restrict__ root, const uint4 *__restrict__ sched,
+                         const uint32_t *__restrict__ order, uint32_t m, uint32_t split,
+                         uint32_t *__restrict__ nitems, const uint32_t *__restrict__ off,
+                         uint4 *__restrict__ items, uint32_t *__restrict__ item_c,
+                         unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
+{
+    extern __shared__ uint4 lds[];
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const uint32_t c = order ? order[q] : q;
+    const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
+    const Sched S = load_sched(sched, c);
+    const uint4 r = root[c];
+    LdsStack st{lds + threadIdx.x, blockDim.x};
+    uint64_t cnt = 0, nn = 0;
+    uint32_t n = 0;
+    const uint32_t base = WRITE ? off[q] : 0u;
+    if (r.w != 0u) {
+        if (S.T < 2u || split == 0u || split > S.T - 2u) {
+            // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
+            if (WRITE) { items[base] = r; item_c[base] = c; }
+            n = 1;
+        } else {
+            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) {
+                if (WRITE) {
+                    items[base + n] = make_uint4(a, b, cc, f | (split << 19));
+                    item_c[base + n] = c;
+                }
+                n++;
+            };
+            dfs_subtree<NODES, true>(mask, nsh, S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
+        }
+    }
+    if (!WRITE) nitems[q] = n;
+    if (WRITE && NODES) nodes_out[c] = nn;   // expand nodes; DFS nodes are added atomically
+    (void)raw;
+}
+
+// exclusive scan of nitems[0..m) -> off, total -> *count (single block of 1024)
+__global__ void k_scan(const uint32_t *__restrict__ nitems, uint32_t m, uint32_t *__restrict__ off,
+                       unsigned long long *__restrict__ count)
+{
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (m + 1023u) / 1024u;
+    const uint32_t lo = t * per, hi = min(m, lo + per);
+    uint64_t s = 0;
+    for (uint32_t a = lo; a < hi; a++) s += nitems[a];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024u; o <<= 1) {
+        uint64_t v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[t] - s;
+    for (uint32_t a = lo; a < hi; a++) { off[a] = (uint32_t)run; run += nitems[a]; }
+    if (t == 1023u) *count = part[1023];
+}
+
+// ---------------------------------------------------------------------------
+// persistent batched kernel: every lane runs its own DFS and takes the next
+// work item from the wave's claimed range as soon as its subtree is done;
+// the wave claims `chunk` items at a time from the global queue head.
+// ---------------------------------------------------------------------------
+template <bool NODES, bool LDS_STACK>
+__global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t *__restrict__ item_c,
+                          const uint4 *__restrict__ sched, const unsigned long long *__restrict__ nitems_total,
+                          unsigned int *__restrict__ head, uint32_t chunk,
+                          unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
+{
+    extern __shared__ uint4 lds[];
+    const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
+    const uint32_t n_items = (uint32_t)*nitems_total;
+    const uint32_t lane = lane_id();
+    LdsStack lst{lds + threadIdx.x, blockDim.x};
+    PrivStack pst;
+    uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
+    bool exhausted = false;           // wave-uniform: queue drained
+    bool have = false;                // this lane holds a subtree
+    uint32_t c = 0, ld = 0, rd = 0, col = 0, avail = 0, d = 0, p = 0;
+    int sp = 0;
+    Sched S{};
+    uint64_t cnt = 0, nn = 0;
+    for (;;) {
+        const uint64_t need = __ballot(!have);
+        if (need != 0ull && !exhausted) {
+            if (next >= end) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(head, chunk);
+                b = __shfl(b, 0, 64);
+                if (b >= n_items) exhausted = true;
+                else { next = b; end = min(b + chunk, n_items); }
+            }
+            if (!exhausted) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+                const uint32_t idx = next + rank;
+                if (!have && idx < end) {
+                    const uint4 it = items[idx];
+                    c = item_c[idx];
+                    S = load_sched(sched, c);
+                    d = it.w >> 19;
+                    avail = it.w & 0x7FFFFu;
+                    ld = it.x; rd = it.y; col = it.z;
+                    cnt = 0; nn = 0; sp = 0;
+                    if (avail != 0u) {
+                        if (S.T == 0u) {
+                            cnt = (avail & S.tm) ? 1u : 0u;
+                        } else if (d + 1u == S.T) {
+                            cnt = term_count(ld, rd, col, avail, pbyte(S, d), S.tm, nsh);
+                            if (NODES) nn = (uint32_t)__builtin_popcount(avail);
+                        } else {
+                            p = pbyte(S, d);
+                            have = true;
+                        }
+                    }
+                    if (!have) {
+                        if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
+                        if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
+                    }
+                }
+                next = min(next + (uint32_t)__builtin_popcountll(need), end);
+            }
+        }
+        if (exhausted && __ballot(have) == 0ull) break;
+        if (!have) continue;
+        // ---- one DFS node (the loop body of dfs_subtree) ----
+        const uint32_t s = (p & 3u) + 1u;
+        const uint32_t bit = avail & (0u - avail);
+        avail ^= bit;
+        uint32_t nld = ((ld | bit) << s) | ((p >> 2) & 3u);
+        const uint32_t nrd = ((rd | bit) >> s) | (((p >> 4) & 3u) << nsh);
+        const uint32_t ncol = col | bit;
+        uint32_t nf = mask & ~(nld | nrd | ncol);
+        const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
+        const uint32_t cd = d + 1u;
+        const uint32_t cp = pbyte(S, cd);
+        bool ok = (nf != 0u) && (((p & 0x40u) == 0u) || (look != 0u));
+        if (cp & 0x80u) { nf &= ~1u; nld |= 1u; }
+        ok = ok && (nf != 0u);
+        if (NODES) nn += 1u;
+        if (cd == S.T - 1u) {
+            if (ok) {
+                cnt += term_count(nld, nrd, ncol, nf, cp, S.tm, nsh);
+                if (NODES) nn += (uint32_t)__builtin_popcount(nf);
+            }
+            ok = false;
+        }
+        if (ok) {
+            if (avail) {
+                const uint4 e = pack_frame(ld, rd, col, avail, d, p);
+                if (LDS_STACK) lst.put(sp, e); else pst.put(sp, e);
+                sp++;
+            }
+            ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; p = cp;
+        } else if (avail == 0u) {
+            if (sp == 0) {
+                if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
+                if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
+                have = false;
+            } else {
+                sp--;
+                const uint4 e = LDS_STACK ? lst.get(sp) : pst.get(sp);
+                ld = e.x; rd = e.y; col = e.z;
+                avail = e.w & 0x7FFFFu;
+                d = (e.w >> 19) & 31u;
+                p = e.w >> 24;
+            }
+        }
+    }
+}
+
+// weighted[c] = raw[c] << wshift[c]; total = sum
+__global__ void k_finalize(const uint4 *__restrict__ sched, uint32_t m, const unsigned long long *__restrict__ raw,
+                           unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ total)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t v = 0;
+    if (c < m) {
+        v = raw[c] << ((sched[2 * c + 1].y >> 8) & 3u);
+        weighted[c] = v;
+    }
+    v = wave_sum_u64(v);
+    if (lane_id() == 0 && v) atomicAdd(total, (unsigned long long)v);
+}
+
+// fill kernel (used instead of hipMemsetAsync so that the sequence can be
+// captured in a graph)
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, uint64_t n, unsigned long long v)
+{
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        p[a] = v;
+}
+__global__ void k_fill_u32(unsigned int *__restrict__ p, uint64_t n, unsigned int v)
+{
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        p[a] = v;
+}
+
+// ---------------------------------------------------------------------------
+// symmetric solutions: F180 (half-turn invariant) and F90 (quarter-turn)
+// ---------------------------------------------------------------------------
+// A half-turn invariant solution is fixed by its top N/2 rows; a queen at
+// (r,c) implies one at (N-1-r, N-1-c).  The attacked columns/diagonals of a
+// symmetric partial placement are themselves symmetric, so the mirrored
+// queen is free exactly when (r,c) is; only the pair's own conflicts
+// (same column c == N-1-c, same diagonal r == c, same anti-diagonal
+// r + c == N-1) need a separate test.  Each lane owns one prefix of P rows.
+// At a leaf the quarter-turn test q[q[r]] == N-1-r is applied.
+__device__ __forceinline__ uint32_t row_free(int N, int r, uint32_t full, uint32_t cols, uint64_t dg, uint64_t an)
+{
+    uint32_t f = full & ~(cols | (uint32_t)(dg >> (N - 1 - r)) | (uint32_t)(an >> r));
+    f &= ~(1u << r);                 // r == c: pair on one diagonal
+    f &= ~(1u << (N - 1 - r));       // r + c == N-1: pair on one anti-diagonal
+    if (N & 1) f &= ~(1u << (N / 2)); // c == N-1-c
+    return f;
+}
+
+__device__ __forceinline__ bool quarter_turn(int N, int h, uint64_t q)
+{
+    // q holds the column of row r < h in bits 5r..5r+4; rows >= N-h mirror them
+    for (int r = 0; r < N; r++) {
+        int c;
+        if (r < h) c = (int)((q >> (5 * r)) & 31u);
+        else if (r >= N - h) c = N - 1 - (int)((q >> (5 * (N - 1 - r))) & 31u);
+        else c = N / 2;
+        int cc;
+        if (c < h) cc = (int)((q >> (5 * c)) & 31u);
+        else if (c >= N - h) cc = N - 1 - (int)((q >> (5 * (N - 1 - c))) & 31u);
+        else cc = N / 2;
+        if (cc != N - 1 - r) return false;
+    }
+    return true;
+}
+
+__global__ void k_symmetric(int N, int P, uint32_t nprefix, unsigned long long *__restrict__ out2)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t c180 = 0, c90 = 0;
+    const int h = N / 2;
+    if (g < nprefix) {
+        const uint32_t full = (1u << N) - 1u;
+        uint32_t cols = 0;
+        uint64_t dg = 0, an = 0, q = 0;
+        if (N & 1) {
+            const int mid = N / 2;
+            cols = 1u << mid;
+            dg = 1ull << (N - 1);
+            an = 1ull << (2 * mid);
+        }
+        bool ok = true;
+        uint32_t rest = g;
+        for (int r = 0; r < P && ok; r++) {
+            const int c = (int)(rest % (uint32_t)N);
+            rest /= (uint32_t)N;
+            const uint32_t f = row_free(N, r, full, cols, dg, an);
+            if (!((f >> c) & 1u)) { ok = false; break; }
+            cols |= (1u << c) | (1u << (N - 1 - c));
+            dg |= (1ull << (c - r + N - 1)) | (1ull << (r - c + N - 1));
+            an |= (1ull << (r + c)) | (1ull << (2 * N - 2 - r - c));
+            q |= (uint64_t)c << (5 * r);
+        }
+        if (ok) {
+            if (P == h) {
+                c180 = 1;
+                c90 = quarter_turn(N, h, q) ? 1 : 0;
+            } else {
+                // iterative DFS over rows P..h-1
+                uint32_t fst[16];
+                uint32_t cst[16];
+                uint64_t dst[16], ast[16], qst[16];
+                int r = P;
+                uint32_t f = row_free(N, r, full, cols, dg, an);
+                for (;;) {
+                    if (f == 0u) {
+                        if (r == P) break;
+                        r--;
+                        f = fst[r]; cols = cst[r]; dg = dst[r]; an = ast[r]; q = qst[r];
+                        continue;
+                    }
+                    const uint32_t bit = f & (0u - f);
+                    f ^= bit;
+                    const int c = __builtin_ctz(bit);
+                    const uint32_t ncols = cols | bit | (1u << (N - 1 - c));
+                    const uint64_t ndg = dg | (1ull << (c - r + N - 1)) | (1ull << (r - c + N - 1));
+                    const uint64_t nan_ = an | (1ull << (r + c)) | (1ull << (2 * N - 2 - r - c));
+                    const uint64_t nq = q | ((uint64_t)c << (5 * r));
+                    if (r + 1 == h) {
+                        c180++;
+                        if (quarter_turn(N, h, nq)) c90++;
+                        continue;
+                    }
+                    fst[r] = f; cst[r] = cols; dst[r] = dg; ast[r] = an; qst[r] = q;
+                    r++;
+                    cols = ncols; dg = ndg; an = nan_; q = nq;
+                    f = row_free(N, r, full, cols, dg, an);
+                }
+            }
+        }
+    }
+    c180 = wave_sum_u64(c180);
+    c90 = wave_sum_u64(c90);
+    if (lane_id() == 0) {
+        if (c180) atomicAdd(out2 + 1, (unsigned long long)c180);
+        if (c90) atomicAdd(out2 + 0, (unsigned long long)c90);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host)
+// ---------------------------------------------------------------------------
+hipError_t launch_decode(hipStream_t s, int N, const uint4 *rec, uint32_t m, uint4 *root, uint4 *sched, uint32_t *err)
+{
+    if (m == 0) return hipSuccess;
+    k_decode<<<(m + 255) / 256, 256, 0, s>>>(N, rec, m, root, sched, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, int N, const uint4 *root,
+                        const uint4 *sched, uint32_t m, unsigned long long *weighted, unsigned long long *nodes_out,
+                        unsigned long long *total)
+{
+    if (m == 0) return hipSuccess;
+    const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
+    const dim3 grid((m + block - 1) / block);
+    if (nodes) k_plain<true><<<grid, block, lds, s>>>(N, root, sched, m, weighted, nodes_out, total, stack_depth);
+    else k_plain<false><<<grid, block, lds, s>>>(N, root, sched, m, weighted, nodes_out, total, stack_depth);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth, int N,
+                         const uint4 *root, const uint4 *sched, const uint32_t *order, uint32_t m, uint32_t split,
+                         uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
+                         unsigned long long *raw, unsigned long long *nodes_out)
+{
+    if (m == 0) return hipSuccess;
+    const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
+    const dim3 grid((m + block - 1) / block);
+#define NQ_EXPAND(NB, WB) \
+    k_expand<NB, WB><<<grid, block, lds, s>>>(N, root, sched, order, m, split, nitems, off, items, item_c, raw, nodes_out)
+    if (write) { if (nodes) NQ_EXPAND(true, true); else NQ_EXPAND(false, true); }
+    else { if (nodes) NQ_EXPAND(true, false); else NQ_EXPAND(false, false); }
+#undef NQ_EXPAND
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count)
+{
+    k_scan<<<1, 1024, 0, s>>>(nitems, m, off, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, uint32_t stack_depth,
+                          int N, const uint4 *items, const uint32_t *item_c, const uint4 *sched,
+                          const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
+                          unsigned long long *raw, unsigned long long *nodes_out)
+{
+    const size_t lds = lds_stack ? (size_t)block * stack_depth * sizeof(uint4) : 0;
+#define NQ_BATCHED(NB, LB) \
+    k_batched<NB, LB><<<grid, block, lds, s>>>(N, items, item_c, sched, n_items, head, chunk, raw, nodes_out)
+    if (lds_stack) { if (nodes) NQ_BATCHED(true, true); else NQ_BATCHED(false, true); }
+    else { if (nodes) NQ_BATCHED(true, false); else NQ_BATCHED(false, false); }
+#undef NQ_BATCHED
+    return hipGetLastError();
+}
+
+hipError_t batched_occupancy(bool nodes, bool lds_stack, int block, size_t lds_bytes, int *blocks_per_cu)
+{
+    const void *f;
+    if (lds_stack) f = nodes ? (const void *)k_batched<true, true> : (const void *)k_batched<false, true>;
+    else f = nodes ? (const void *)k_batched<true, false> : (const void *)k_batched<false, false>;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, lds_bytes);
+}
+
+hipError_t launch_finalize(hipStream_t s, const uint4 *sched, uint32_t m, const unsigned long long *raw,
+                           unsigned long long *weighted, unsigned long long *total)
+{
+    if (m == 0) return hipSuccess;
+    k_finalize<<<(m + 255) / 256, 256, 0, s>>>(sched, m, raw, weighted, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, uint64_t n, unsigned long long v)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_fill_u64<<<(uint32_t)blocks, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(hipStream_t s, unsigned int *p, uint64_t n, unsigned int v)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_fill_u32<<<(uint32_t)blocks, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_symmetric(hipStream_t s, int N, unsigned long long *out2)
+{
+    const int h = N / 2;
+    const int P = h < 3 ? h : 3;
+    uint32_t np = 1;
+    for (int a = 0; a < P; a++) np *= (uint32_t)N;
+    k_symmetric<<<(np + 255) / 256, 256, 0, s>>>(N, P, np, out2);
+    return hipGetLastError();
+}
+
+}  // namespace nqk

@@ -1,0 +1,117 @@
+"""count(N) -> (total, unique): the constellation solver's public API.
+
+Mirrors the reference's per-N driver (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:2116-2160):
+generate the constellations (gen_constellations, :2139), count each one
+(exec_solutions, :2153 -> :1057-1275) and sum the weighted counts (:2156).
+Counting runs on the GPU through libnq; there is no host fallback.
+
+Unique: the reference's constellation solvers print 0 (:2160).  Unique here
+is the number of solution orbits under the 8 board symmetries, the quantity
+the reference's bit-symmetry solver reports
+(13Bit_codon/07Py_bit_symmetry_codon.py:410-411, unique = c2 + c4 + c8).
+With F90 and F180 the solutions fixed by the quarter and half turns, and no
+solution fixed by a reflection, Burnside's lemma gives
+unique = (total + F180 + 2 * F90) / 8; F90 and F180 come from a GPU
+enumerator of rotation-symmetric solutions.
+"""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+
+from . import constellations as _cons
+from ._lib import Context, Sched, as_records
+
+__all__ = ["count", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
+           "preset_for", "MIN_N", "MAX_N"]
+
+MIN_N, MAX_N = 6, 21
+
+_ctx: dict[int, Context] = {}
+_ctx_lock = threading.Lock()
+
+
+def default_context(device: int = 0) -> Context:
+    """A process-wide context per device (created on first use)."""
+    with _ctx_lock:
+        c = _ctx.get(device)
+        if c is None:
+            c = Context(device)
+            _ctx[device] = c
+        return c
+
+
+def preset_for(N: int) -> int:
+    """Preset-queen depth: 18Py always uses 5 (:2139)."""
+    return 5
+
+
+def default_sched(N: int | None = None) -> Sched:
+    """Batched schedule used when none is given (split 3, 128-lane groups, LDS stack)."""
+    return Sched(split_depth=3, block=128, lds_stack=1, use_graph=0, chunk=64, waves_per_eu=0)
+
+
+def unique_count(total: int, f90: int, f180: int) -> int:
+    num = int(total) + int(f180) + 2 * int(f90)
+    if num % 8:
+        raise ArithmeticError(f"orbit count is not integral: ({total} + {f180} + 2*{f90}) / 8")
+    return num // 8
+
+
+def _check_n(N: int) -> None:
+    if not MIN_N <= int(N) <= MAX_N:
+        raise ValueError(f"N={N} outside the GPU path ({MIN_N}..{MAX_N})")
+
+
+def count_tasks(N: int, records, *, mode: str = "batched", sched: Sched | None = None, order=None,
+                nodes: bool = False, ctx: Context | None = None):
+    """Count a list of constellation records on the GPU.
+
+    Returns (weighted per record u64 array, node counts or None, total).
+    `mode` is "plain" (one lane per constellation) or "batched" (split into
+    work items, persistent kernel).  `order` is the batched processing order."""
+    _check_n(N)
+    ctx = ctx or default_context()
+    rec = as_records(records)
+    ctx.load(N, rec, order)
+    if mode == "plain":
+        total, _ = ctx.run_plain(count_nodes=nodes)
+    elif mode == "batched":
+        total, _ = ctx.run_batched(sched or default_sched(N), count_nodes=nodes)
+    else:
+        raise ValueError(f"unknown mode {mode!r}")
+    w, n = ctx.results(weighted=True, nodes=nodes)
+    return w, n, total
+
+
+def exec_solutions(N: int, constellations: list, use_gpu: bool = True) -> None:
+    """exec_solutions(N, constellations, use_gpu) of 18Py:1057: writes each
+    dict's "solutions" (weighted count) in place.  Only the GPU path exists."""
+    if not use_gpu:
+        raise ValueError("the CPU path of exec_solutions is not provided; use_gpu must be True")
+    if not constellations:
+        return
+    rec = np.array([[c["ld"], c["rd"], c["col"], c["startijkl"]] for c in constellations], dtype=np.uint64)
+    if (rec >> 32).any():
+        raise OverflowError("a record field does not fit 32 bits")
+    w, _, _ = count_tasks(N, rec.astype(np.uint32))
+    for c, v in zip(constellations, w):
+        c["solutions"] = int(v)
+
+
+def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
+          sched: Sched | None = None, ctx: Context | None = None) -> tuple[int, int]:
+    """Total and unique N-Queens solutions for 6 <= N <= 21, counted on the GPU."""
+    _check_n(N)
+    ctx = ctx or default_context()
+    rec = _cons.gen_constellations(N, preset or preset_for(N))
+    order = None
+    if scheduler == "mlp":
+        from .costmodel import default_model
+        order = default_model().order(N, rec, ctx=ctx)
+    elif scheduler != "fifo":
+        raise ValueError(f"unknown scheduler {scheduler!r}")
+    _, _, total = count_tasks(N, rec, mode=mode, sched=sched, order=order, ctx=ctx)
+    f90, f180 = ctx.symmetric_counts(N)
+    return total, unique_count(total, f90, f180)

@@ -1,0 +1,139 @@
+"""GPU parity: libnq (through its C ABI) against the oracle and the
+reference's fixtures.  Bit-exact everywhere: weighted counts, node counts,
+totals and unique counts."""
+import numpy as np
+import pytest
+
+import nqueens
+from nqueens import constellations as C
+from tests.helpers import solutions_sha256
+
+pytestmark = pytest.mark.gpu
+
+
+def sched(split=3, block=128, lds=1, graph=0, chunk=64):
+    return nqueens.Sched(split_depth=split, block=block, lds_stack=lds, use_graph=graph, chunk=chunk,
+                         waves_per_eu=0)
+
+
+def test_device_info(ctx):
+    info = ctx.device_info()
+    print(info, flush=True)
+    assert info["wavefront"] == 64 and info["compute_units"] > 0
+
+
+@pytest.mark.parametrize("N", range(6, 18))
+def test_plain_preset4_per_record(ctx, oracle, N):
+    rec = C.gen_constellations(N, 4)
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    total, _ = ctx.run_plain(count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, w)
+    assert np.array_equal(gn, nodes)
+    assert total == int(w.sum())
+
+
+@pytest.mark.parametrize("N", range(6, 20))
+def test_plain_preset5_fixture_digest(ctx, digests, N):
+    rec = C.gen_constellations(N, 5)
+    ctx.load(N, rec)
+    total, _ = ctx.run_plain()
+    gw, _ = ctx.results()
+    d = digests[str(N)]
+    assert total == d["solutions_sum"]
+    assert solutions_sha256(rec, gw) == d["solutions_sha256"]
+
+
+@pytest.mark.parametrize("split", [0, 1, 2, 3, 5])
+@pytest.mark.parametrize("block,lds", [(64, 1), (128, 1), (256, 1), (64, 0), (256, 0)])
+def test_batched_per_record(ctx, oracle, split, block, lds):
+    N = 13
+    rec = C.gen_constellations(N, 5)
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    total, t = ctx.run_batched(sched(split, block, lds), count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, w)
+    assert np.array_equal(gn, nodes)
+    assert total == int(w.sum())
+    assert t.items >= 1
+
+
+@pytest.mark.parametrize("N,preset", [(14, 4), (15, 6), (16, 5), (17, 4)])
+def test_batched_other_presets(ctx, oracle, N, preset):
+    rec = C.gen_constellations(N, preset)
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    total, _ = ctx.run_batched(sched(3, 128, 1), count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+
+
+def test_batched_order_is_schedule_only(ctx, oracle):
+    N = 14
+    rec = C.gen_constellations(N, 5)
+    w, _, _ = oracle.count_records(N, rec)
+    perm = np.random.default_rng(7).permutation(rec.shape[0]).astype(np.uint32)
+    ctx.load(N, rec, order=perm)
+    total, _ = ctx.run_batched(sched(2, 64, 1))
+    gw, _ = ctx.results()
+    assert np.array_equal(gw, w)
+
+
+def test_batched_graph_replays(ctx, oracle):
+    """The captured sequence replayed several times, with eager runs between."""
+    N = 14
+    rec = C.gen_constellations(N, 5)
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    for rep in range(4):
+        total, _ = ctx.run_batched(sched(3, 128, 1, graph=1), count_nodes=True)
+        gw, gn = ctx.results(weighted=True, nodes=True)
+        assert np.array_equal(gw, w) and np.array_equal(gn, nodes), rep
+        t2, _ = ctx.run_plain()
+        assert t2 == int(w.sum())
+
+
+@pytest.mark.parametrize("N", [17, 18, 19])
+def test_batched_fixture_digest(ctx, digests, N):
+    rec = C.gen_constellations(N, 5)
+    ctx.load(N, rec)
+    total, _ = ctx.run_batched(sched(3, 128, 1))
+    gw, _ = ctx.results()
+    assert total == digests[str(N)]["solutions_sum"]
+    assert solutions_sha256(rec, gw) == digests[str(N)]["solutions_sha256"]
+
+
+@pytest.mark.parametrize("N", range(4, 22))
+def test_symmetric_counts(ctx, oracle, N):
+    assert ctx.symmetric_counts(N) == oracle.symmetric_counts(N)
+
+
+@pytest.mark.parametrize("N", range(6, 19))
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_count_api_matches_published_table(known, N, mode):
+    assert list(nqueens.count(N, mode=mode)) == known["readme_total_unique"][str(N)]
+
+
+def test_exec_solutions_fills_dicts(p5_full):
+    rec, sol = p5_full["rec11"], p5_full["sol11"]
+    cs = [{"ld": int(r[0]), "rd": int(r[1]), "col": int(r[2]), "startijkl": int(r[3]), "solutions": 0} for r in rec]
+    nqueens.exec_solutions(11, cs)
+    assert [c["solutions"] for c in cs] == sol.tolist()
+
+
+def test_empty_set_and_errors(ctx):
+    ctx.load(12, np.zeros((0, 4), np.uint32))
+    assert ctx.run_plain()[0] == 0
+    assert ctx.run_batched(sched())[0] == 0
+    with pytest.raises(nqueens.NQError):
+        ctx.load(22, C.gen_constellations(12, 5))
+    # a P5 target (functionid 8..11) is refused, never counted
+    bad = np.array([[0, 0, 0, (2 << 20) | C.to_ijkl(5, 3, 7, 9)]], np.uint32)
+    assert nqueens.decode_host(21, bad)[0, 9] == 1
+    with pytest.raises(nqueens.NQError) as e:
+        ctx.load(21, bad)
+    assert e.value.code == -2
+    with pytest.raises(nqueens.NQError):
+        ctx.load(12, C.gen_constellations(12, 5), order=np.zeros(4300, np.uint32))

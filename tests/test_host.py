@@ -1,0 +1,118 @@
+"""Host side of the product (CPU only): generator, record decode, formats and
+the C-ABI library's exports.  No GPU call is made here."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import nqueens
+from nqueens import constellations as C
+from tests.helpers import records_sha256
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+@pytest.mark.parametrize("N", range(6, 22))
+def test_generator_matches_fixture(digests, N):
+    rec = C.gen_constellations(N, 5)
+    assert rec.dtype == np.uint32 and rec.shape == (digests[str(N)]["constellations"], 4)
+    assert records_sha256(rec) == digests[str(N)]["records_sha256"]
+
+
+@pytest.mark.parametrize("N,preset", [(8, 4), (12, 4), (13, 4), (17, 4), (12, 6), (16, 6), (14, 7), (11, 3)])
+def test_generator_matches_oracle(oracle, N, preset):
+    mine = C.gen_constellations(N, preset)
+    theirs, _ = oracle.gen_constellations(N, preset, scope=oracle.SCOPE_NONE)
+    assert records_sha256(mine) == records_sha256(theirs)
+
+
+def test_generator_preset_counts(known):
+    pc = known["preset_record_counts"]
+    assert C.gen_constellations(18, 6).shape[0] == pc["18,6"]
+    assert C.gen_constellations(21, 6).shape[0] == pc["21,6"]
+    assert C.gen_constellations(18, 7).shape[0] == pc["18,7"]
+
+
+@pytest.mark.parametrize("N", [7, 8, 11, 12, 16])
+def test_jasmin_and_weights_match_oracle(oracle, N):
+    for i in range(N):
+        for j in range(N):
+            for k in range(N):
+                for l in range(0, N, 3):
+                    x = C.to_ijkl(i, j, k, l)
+                    assert C.jasmin(x, N) == oracle.jasmin(x, N)
+                    assert C.symmetry(x, N) == oracle.symmetry(x, N)
+
+
+def test_symmetry_closed_forms():
+    N = 9
+    for x in [C.to_ijkl(1, 7, 2, 6), C.to_ijkl(0, 4, 0, 7), C.to_ijkl(3, 5, 2, 6)]:
+        assert C.rot90(C.rot90(C.rot90(C.rot90(x, N), N), N), N) == x
+        assert C.rot180(x, N) == C.rot90(C.rot90(x, N), N)
+        assert C.mirvert(C.mirvert(x, N), N) == x
+        assert C.jasmin(C.jasmin(x, N), N) == C.jasmin(x, N)
+    # weight 2: a quarter-turn fixed point (i,j,k,l) = (N-1-k, N-1-l, j, i)
+    x = C.to_ijkl(2, 6, 6, 2)
+    assert C.rot90(x, N) == x and C.symmetry(x, N) == 2
+    x = C.to_ijkl(1, 7, 3, 5)       # half-turn fixed only
+    assert C.symmetry(x, N) == 4
+
+
+@pytest.mark.parametrize("N,preset", [(n, 5) for n in range(6, 22)] + [(n, 4) for n in (8, 13, 17, 21)] +
+                         [(n, 6) for n in (12, 17)])
+def test_decode_matches_oracle_build_soa(oracle, N, preset):
+    rec = C.gen_constellations(N, preset)
+    d = nqueens.decode_host(N, rec)
+    o = oracle.build_soa(N, rec)
+    assert (d[:, 9] == 0).all(), "decode status"
+    assert np.array_equal(d[:, :7], o[:, :7])                  # fid, start, jmark, endmark, mark1, mark2, w
+    assert np.array_equal(d[:, 7].astype(np.uint32), o[:, 7].astype(np.uint32))  # free
+    # the terminal depth is constant: N-1-preset rows are left to fill
+    assert set(np.unique(d[:, 8]).tolist()) == {N - 1 - preset}
+
+
+def test_decode_rejects_out_of_range():
+    rec = C.gen_constellations(12, 5)
+    with pytest.raises(nqueens.NQError):
+        nqueens.decode_host(22, rec)
+    bad = rec.copy()
+    bad[:, 3] = (31 << 20) | (bad[:, 3] & 0xFFFFF)   # start row beyond the board
+    d = nqueens.decode_host(12, bad)
+    assert (d[:, 9] != 0).all()
+
+
+def test_bin_formats_roundtrip(tmp_path, p5_full):
+    rec, sol = p5_full["rec10"], p5_full["sol10"]
+    C.write_constellations_bin(tmp_path / "c.bin", rec)
+    assert np.array_equal(C.read_constellations_bin(tmp_path / "c.bin"), rec)
+    C.write_solutions_bin(tmp_path / "s.bin", rec, sol)
+    r2, s2 = C.read_solutions_bin(tmp_path / "s.bin")
+    assert np.array_equal(r2, rec) and np.array_equal(s2, sol)
+    (tmp_path / "bad.bin").write_bytes(b"\0" * 17)
+    with pytest.raises(ValueError):
+        C.read_constellations_bin(tmp_path / "bad.bin")
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "nq.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(nq_\w+)\s*\(", header, re.M))
+    assert len(declared) >= 15
+    lib = nqueens.load_library()
+    for name in declared:
+        assert hasattr(lib, name), name
+    from nqueens._lib import SIGNATURES
+    assert declared == set(SIGNATURES)
+    assert lib.nq_abi_version() == 1
+
+
+def test_solver_argument_checks():
+    with pytest.raises(ValueError):
+        nqueens.count(5)
+    with pytest.raises(ValueError):
+        nqueens.count(22)
+    with pytest.raises(ValueError):
+        nqueens.exec_solutions(8, [{"ld": 0, "rd": 0, "col": 0, "startijkl": 0}], use_gpu=False)
+    with pytest.raises(ArithmeticError):
+        nqueens.unique_count(10, 0, 1)
+    assert nqueens.unique_count(14200, 8, 80) == 1787
