@@ -10,16 +10,17 @@
 #include <vector>
 #include <algorithm>
 
-#include "../../include/nq.h"
 #define NQ_HD __host__ __device__
 #include "nq_task.h"
 #include "nq_kernels.h"
 
+#include "nq_ctx.h"
+
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char *fmt, ...)
+int nq_fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -30,75 +31,16 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(expr)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(NQ_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                             \
-    } while (0)
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    hipError_t reserve(size_t want)
-    {
-        if (want <= n && p) return hipSuccess;
-        if (p) { hipError_t e = hipFree(p); p = nullptr; n = 0; if (e != hipSuccess) return e; }
-        size_t cap = want ? want : 1;
-        hipError_t e = hipMalloc((void **)&p, cap * sizeof(T));
-        if (e == hipSuccess) n = cap;
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr; n = 0;
-    }
-};
-
-struct GraphKey {
-    int split = -1, block = 0, lds = 0, chunk = 0, grid = 0, nodes = 0;
-    bool operator==(const GraphKey &o) const
-    {
-        return split == o.split && block == o.block && lds == o.lds && chunk == o.chunk && grid == o.grid &&
-               nodes == o.nodes;
-    }
-};
-
-}  // namespace
-
-struct nq_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    int cu = 0, lds_max = 0, wave = 64, clock_khz = 0;
-    // resident task set
-    int N = 0;
-    uint64_t m = 0;
-    uint32_t Tmax = 0;
-    bool loaded = false;
-    bool last_nodes = false;
-    DevBuf<uint4> rec, root, sched, items;
-    DevBuf<uint32_t> order, nitems, off, item_c, err, head;
-    DevBuf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
-    int sized_split = -1;
-    uint64_t item_total = 0;
-    // captured batched sequence
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    GraphKey gkey;
-};
+int nq_check_ctx(nq_ctx *c)
+{
+    if (!c) return nq_fail(NQ_EINVAL, "null context");
+    NQ_HIPCHK(hipSetDevice(c->device));
+    return 0;
+}
 
 namespace {
 
-int check_ctx(nq_ctx *c)
-{
-    if (!c) return fail(NQ_EINVAL, "null context");
-    HIPCHK(hipSetDevice(c->device));
-    return 0;
-}
+int check_ctx(nq_ctx *c) { return nq_check_ctx(c); }
 
 void drop_graph(nq_ctx *c)
 {
@@ -106,7 +48,7 @@ void drop_graph(nq_ctx *c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
     c->graph = nullptr;
-    c->gkey = GraphKey{};
+    c->gkey = nq_graph_key{};
 }
 
 // LDS entries per lane: frames with remaining candidates sit at depths
@@ -116,7 +58,7 @@ uint32_t stack_entries(const nq_ctx *c) { return c->Tmax > 1 ? c->Tmax - 1 : 1; 
 int elapsed(nq_ctx *c, int a, int b, double *ms)
 {
     float f = 0.f;
-    HIPCHK(hipEventElapsedTime(&f, c->ev[a], c->ev[b]));
+    NQ_HIPCHK(hipEventElapsedTime(&f, c->ev[a], c->ev[b]));
     *ms = f;
     return 0;
 }
@@ -125,16 +67,16 @@ int elapsed(nq_ctx *c, int a, int b, double *ms)
 int size_items(nq_ctx *c, int split, int block)
 {
     const uint32_t sd = stack_entries(c);
-    HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->N, c->root.p, c->sched.p,
+    NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->N, c->root.p, c->sched.p,
                               c->order.p, (uint32_t)c->m, (uint32_t)split, c->nitems.p, nullptr, nullptr,
                               nullptr, nullptr, nullptr));
-    HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p));
+    NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p));
     unsigned long long cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (cnt >= (1ull << 32)) return fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^32)", split, cnt);
-    HIPCHK(c->items.reserve(cnt));
-    HIPCHK(c->item_c.reserve(cnt));
+    NQ_HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    if (cnt >= (1ull << 32)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^32)", split, cnt);
+    NQ_HIPCHK(c->items.reserve(cnt));
+    NQ_HIPCHK(c->item_c.reserve(cnt));
     c->item_total = cnt;
     c->sized_split = split;
     return 0;
@@ -146,24 +88,24 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
 {
     const uint32_t m = (uint32_t)c->m, sd = stack_entries(c);
     hipStream_t st = c->stream;
-    if (events) HIPCHK(hipEventRecord(c->ev[0], st));
-    HIPCHK(nqk::launch_fill_u64(st, c->raw.p, m, 0));
-    HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
-    HIPCHK(nqk::launch_fill_u32(st, c->head.p, 1, 0));
-    if (!nodes) HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, m, 0));
-    HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->N, c->root.p, c->sched.p, c->order.p, m,
+    if (events) NQ_HIPCHK(hipEventRecord(c->ev[0], st));
+    NQ_HIPCHK(nqk::launch_fill_u64(st, c->raw.p, m, 0));
+    NQ_HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
+    NQ_HIPCHK(nqk::launch_fill_u32(st, c->head.p, 1, 0));
+    if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, m, 0));
+    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->N, c->root.p, c->sched.p, c->order.p, m,
                               (uint32_t)s.split_depth, c->nitems.p, nullptr, nullptr, nullptr, nullptr, nullptr));
-    HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
-    HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->N, c->root.p, c->sched.p, c->order.p, m,
+    NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
+    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->N, c->root.p, c->sched.p, c->order.p, m,
                               (uint32_t)s.split_depth, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
                               c->raw.p, c->nodes.p));
-    if (events) HIPCHK(hipEventRecord(c->ev[1], st));
-    HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sd, c->N, c->items.p,
+    if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
+    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sd, c->N, c->items.p,
                                c->item_c.p, c->sched.p, c->item_count.p, c->head.p, (uint32_t)s.chunk,
                                c->raw.p, c->nodes.p));
-    if (events) HIPCHK(hipEventRecord(c->ev[2], st));
-    HIPCHK(nqk::launch_finalize(st, c->sched.p, m, c->raw.p, c->weighted.p, c->total.p));
-    if (events) HIPCHK(hipEventRecord(c->ev[3], st));
+    if (events) NQ_HIPCHK(hipEventRecord(c->ev[2], st));
+    NQ_HIPCHK(nqk::launch_finalize(st, c->sched.p, m, c->raw.p, c->weighted.p, c->total.p));
+    if (events) NQ_HIPCHK(hipEventRecord(c->ev[3], st));
     return 0;
 }
 
@@ -177,28 +119,28 @@ const char *nq_last_error(void) { return g_err.c_str(); }
 
 int nq_ctx_create(int device, nq_ctx **out)
 {
-    if (!out) return fail(NQ_EINVAL, "null out pointer");
+    if (!out) return nq_fail(NQ_EINVAL, "null out pointer");
     *out = nullptr;
     int ndev = 0;
-    HIPCHK(hipGetDeviceCount(&ndev));
-    if (device < 0 || device >= ndev) return fail(NQ_EINVAL, "device %d out of range (%d devices)", device, ndev);
-    HIPCHK(hipSetDevice(device));
+    NQ_HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return nq_fail(NQ_EINVAL, "device %d out of range (%d devices)", device, ndev);
+    NQ_HIPCHK(hipSetDevice(device));
     nq_ctx *c = new nq_ctx;
     c->device = device;
     hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, device));
+    NQ_HIPCHK(hipGetDeviceProperties(&prop, device));
     c->cu = prop.multiProcessorCount;
     c->lds_max = (int)prop.sharedMemPerBlock;
     c->wave = prop.warpSize;
     c->clock_khz = prop.clockRate;
-    if (c->wave != 64) { delete c; return fail(NQ_EUNSUPPORTED, "wavefront size %d, kernels assume 64", prop.warpSize); }
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
-    HIPCHK(c->total.reserve(1));
-    HIPCHK(c->err.reserve(1));
-    HIPCHK(c->head.reserve(1));
-    HIPCHK(c->item_count.reserve(1));
-    HIPCHK(c->sym.reserve(2));
+    if (c->wave != 64) { delete c; return nq_fail(NQ_EUNSUPPORTED, "wavefront size %d, kernels assume 64", prop.warpSize); }
+    NQ_HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) NQ_HIPCHK(hipEventCreate(&e));
+    NQ_HIPCHK(c->total.reserve(1));
+    NQ_HIPCHK(c->err.reserve(1));
+    NQ_HIPCHK(c->head.reserve(1));
+    NQ_HIPCHK(c->item_count.reserve(1));
+    NQ_HIPCHK(c->sym.reserve(2));
     *out = c;
     return 0;
 }
@@ -209,9 +151,6 @@ int nq_ctx_destroy(nq_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (auto *b : {&c->rec, &c->root, &c->sched, &c->items}) b->release();
-    for (auto *b : {&c->order, &c->nitems, &c->off, &c->item_c, &c->err, &c->head}) b->release();
-    for (auto *b : {&c->weighted, &c->nodes, &c->raw, &c->total, &c->item_count, &c->sym}) b->release();
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -221,7 +160,7 @@ int nq_ctx_destroy(nq_ctx *c)
 int nq_device_info(nq_ctx *c, int32_t *out4)
 {
     if (int r = check_ctx(c)) return r;
-    if (!out4) return fail(NQ_EINVAL, "null out pointer");
+    if (!out4) return nq_fail(NQ_EINVAL, "null out pointer");
     out4[0] = c->cu; out4[1] = c->lds_max; out4[2] = c->wave; out4[3] = c->clock_khz;
     return 0;
 }
@@ -229,13 +168,13 @@ int nq_device_info(nq_ctx *c, int32_t *out4)
 int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32_t *order)
 {
     if (int r = check_ctx(c)) return r;
-    if (N < nqt::kMinN || N > nqt::kMaxN) return fail(NQ_EINVAL, "N=%d outside the u32 path (6..21)", N);
-    if (m && !rec4) return fail(NQ_EINVAL, "null record pointer");
-    if (m >= (1ull << 31)) return fail(NQ_EINVAL, "too many records (%llu)", (unsigned long long)m);
+    if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside the u32 path (6..21)", N);
+    if (m && !rec4) return nq_fail(NQ_EINVAL, "null record pointer");
+    if (m >= (1ull << 31)) return nq_fail(NQ_EINVAL, "too many records (%llu)", (unsigned long long)m);
     if (order) {
         std::vector<uint8_t> seen(m, 0);
         for (uint64_t a = 0; a < m; a++) {
-            if (order[a] >= m || seen[order[a]]) return fail(NQ_EINVAL, "order is not a permutation of 0..m-1");
+            if (order[a] >= m || seen[order[a]]) return nq_fail(NQ_EINVAL, "order is not a permutation of 0..m-1");
             seen[order[a]] = 1;
         }
     }
@@ -243,15 +182,15 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
     c->loaded = false;
     c->sized_split = -1;
     const size_t mm = m ? m : 1;
-    HIPCHK(c->rec.reserve(mm));
-    HIPCHK(c->root.reserve(mm));
-    HIPCHK(c->sched.reserve(2 * mm));
-    HIPCHK(c->order.reserve(mm));
-    HIPCHK(c->nitems.reserve(mm));
-    HIPCHK(c->off.reserve(mm));
-    HIPCHK(c->weighted.reserve(mm));
-    HIPCHK(c->nodes.reserve(mm));
-    HIPCHK(c->raw.reserve(mm));
+    NQ_HIPCHK(c->rec.reserve(mm));
+    NQ_HIPCHK(c->root.reserve(mm));
+    NQ_HIPCHK(c->sched.reserve(2 * mm));
+    NQ_HIPCHK(c->order.reserve(mm));
+    NQ_HIPCHK(c->nitems.reserve(mm));
+    NQ_HIPCHK(c->off.reserve(mm));
+    NQ_HIPCHK(c->weighted.reserve(mm));
+    NQ_HIPCHK(c->nodes.reserve(mm));
+    NQ_HIPCHK(c->raw.reserve(mm));
     c->N = N;
     c->m = m;
     std::vector<uint32_t> ord;
@@ -261,26 +200,26 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
         order = ord.data();
     }
     if (m) {
-        HIPCHK(hipMemcpyAsync(c->rec.p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->order.p, order, m * 4, hipMemcpyHostToDevice, c->stream));
+        NQ_HIPCHK(hipMemcpyAsync(c->rec.p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
+        NQ_HIPCHK(hipMemcpyAsync(c->order.p, order, m * 4, hipMemcpyHostToDevice, c->stream));
     }
-    HIPCHK(nqk::launch_fill_u32(c->stream, c->err.p, 1, 0));
-    HIPCHK(nqk::launch_decode(c->stream, N, c->rec.p, (uint32_t)m, c->root.p, c->sched.p, c->err.p));
-    HIPCHK(nqk::launch_fill_u64(c->stream, c->weighted.p, mm, 0));
-    HIPCHK(nqk::launch_fill_u64(c->stream, c->nodes.p, mm, 0));
+    NQ_HIPCHK(nqk::launch_fill_u32(c->stream, c->err.p, 1, 0));
+    NQ_HIPCHK(nqk::launch_decode(c->stream, N, c->rec.p, (uint32_t)m, c->root.p, c->sched.p, c->err.p));
+    NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->weighted.p, mm, 0));
+    NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->nodes.p, mm, 0));
     uint32_t err = 0;
-    HIPCHK(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (err) return fail(NQ_EUNSUPPORTED, "record decode failed (status mask 0x%x: 2=P5 target, 4=shift, "
+    NQ_HIPCHK(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    if (err) return nq_fail(NQ_EUNSUPPORTED, "record decode failed (status mask 0x%x: 2=P5 target, 4=shift, "
                                           "8=no terminal row, 16=depth)", err);
     // the terminal depth is N-1-preset for every record of a set; take the max
     c->Tmax = 0;
     if (m) {
         std::vector<uint4> sc(2 * m);
-        HIPCHK(hipMemcpy(sc.data(), c->sched.p, 2 * m * 16, hipMemcpyDeviceToHost));
+        NQ_HIPCHK(hipMemcpy(sc.data(), c->sched.p, 2 * m * 16, hipMemcpyDeviceToHost));
         for (uint64_t a = 0; a < m; a++) c->Tmax = std::max(c->Tmax, sc[2 * a + 1].y & 0xFFu);
     }
-    if (c->Tmax > (uint32_t)nqk::kStackCap) return fail(NQ_ECAPACITY, "terminal depth %u beyond stack %d", c->Tmax, nqk::kStackCap);
+    if (c->Tmax > (uint32_t)nqk::kStackCap) return nq_fail(NQ_ECAPACITY, "terminal depth %u beyond stack %d", c->Tmax, nqk::kStackCap);
     c->loaded = true;
     return 0;
 }
@@ -288,23 +227,22 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
 int nq_ctx_run_plain(nq_ctx *c, int count_nodes, uint64_t *total_out, nq_timing *t)
 {
     if (int r = check_ctx(c)) return r;
-    if (!c->loaded) return fail(NQ_ESTATE, "no task set loaded");
+    if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
     const int block = 64;
     const uint32_t sd = stack_entries(c);
-    if ((size_t)block * sd * 16 > (size_t)c->lds_max) return fail(NQ_ECAPACITY, "LDS stack too large");
+    if ((size_t)block * sd * 16 > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack too large");
     hipStream_t st = c->stream;
-    HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
-    if (!count_nodes) HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, c->m ? c->m : 1, 0));
-    HIPCHK(hipEventRecord(c->ev[0], st));
-    HIPCHK(hipEventRecord(c->ev[1], st));
-    HIPCHK(nqk::launch_plain(st, count_nodes != 0, block, sd, c->N, c->root.p, c->sched.p, (uint32_t)c->m,
+    NQ_HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
+    if (!count_nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, c->m ? c->m : 1, 0));
+    NQ_HIPCHK(hipEventRecord(c->ev[0], st));
+    NQ_HIPCHK(hipEventRecord(c->ev[1], st));
+    NQ_HIPCHK(nqk::launch_plain(st, count_nodes != 0, block, sd, c->N, c->root.p, c->sched.p, (uint32_t)c->m,
                              c->weighted.p, c->nodes.p, c->total.p));
-    HIPCHK(hipEventRecord(c->ev[2], st));
-    HIPCHK(hipEventRecord(c->ev[3], st));
+    NQ_HIPCHK(hipEventRecord(c->ev[2], st));
+    NQ_HIPCHK(hipEventRecord(c->ev[3], st));
     unsigned long long tot = 0;
-    HIPCHK(hipMemcpyAsync(&tot, c->total.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    c->last_nodes = count_nodes != 0;
+    NQ_HIPCHK(hipMemcpyAsync(&tot, c->total.p, 8, hipMemcpyDeviceToHost, st));
+    NQ_HIPCHK(hipStreamSynchronize(st));
     if (total_out) *total_out = tot;
     if (t) {
         if (int r = elapsed(c, 1, 2, &t->kernel_ms)) return r;
@@ -318,18 +256,18 @@ int nq_ctx_run_plain(nq_ctx *c, int count_nodes, uint64_t *total_out, nq_timing 
 int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t *total_out, nq_timing *t)
 {
     if (int r = check_ctx(c)) return r;
-    if (!c->loaded) return fail(NQ_ESTATE, "no task set loaded");
-    if (!sp) return fail(NQ_EINVAL, "null schedule");
+    if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
+    if (!sp) return nq_fail(NQ_EINVAL, "null schedule");
     nq_sched s = *sp;
-    if (s.block != 64 && s.block != 128 && s.block != 256) return fail(NQ_EINVAL, "block must be 64, 128 or 256");
+    if (s.block != 64 && s.block != 128 && s.block != 256) return nq_fail(NQ_EINVAL, "block must be 64, 128 or 256");
     if (s.chunk < 1) s.chunk = 64;
-    if (s.split_depth < 0) return fail(NQ_EINVAL, "negative split depth");
+    if (s.split_depth < 0) return nq_fail(NQ_EINVAL, "negative split depth");
     // frames at depth T-1 are resolved in place, so items live at depth <= T-2
     const int max_split = c->Tmax >= 2 ? (int)c->Tmax - 2 : 0;
     if (s.split_depth > max_split) s.split_depth = max_split;
     const uint32_t sd = stack_entries(c);
     const size_t lds_bytes = (size_t)s.block * sd * 16;
-    if (lds_bytes > (size_t)c->lds_max) return fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, c->lds_max);
+    if (lds_bytes > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, c->lds_max);
     if (c->m == 0) {
         if (total_out) *total_out = 0;
         if (t) memset(t, 0, sizeof *t);
@@ -340,7 +278,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         if (int r = size_items(c, s.split_depth, s.block)) return r;
     }
     int per_cu = 0;
-    HIPCHK(nqk::batched_occupancy(count_nodes != 0, s.lds_stack != 0, s.block, s.lds_stack ? lds_bytes : 0, &per_cu));
+    NQ_HIPCHK(nqk::batched_occupancy(count_nodes != 0, s.lds_stack != 0, s.block, s.lds_stack ? lds_bytes : 0, &per_cu));
     if (per_cu < 1) per_cu = 1;
     if (s.waves_per_eu > 0) {
         int want = (s.waves_per_eu * 4 * 64) / s.block;   // 4 SIMDs per CU
@@ -350,31 +288,30 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     const int grid = c->cu * per_cu;
     bool timed = true;
     if (s.use_graph) {
-        GraphKey k;
+        nq_graph_key k;
         k.split = s.split_depth; k.block = s.block; k.lds = s.lds_stack; k.chunk = s.chunk; k.grid = grid;
         k.nodes = count_nodes;
         if (!c->gexec || !(k == c->gkey)) {
             drop_graph(c);
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            NQ_HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int r = enqueue_batched(c, s, count_nodes, grid, true);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (r) { if (g) (void)hipGraphDestroy(g); return r; }
-            HIPCHK(e);
+            NQ_HIPCHK(e);
             c->graph = g;
-            HIPCHK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+            NQ_HIPCHK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
             c->gkey = k;
         }
-        HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        NQ_HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     } else {
         if (int r = enqueue_batched(c, s, count_nodes, grid, timed)) return r;
     }
     unsigned long long tot = 0, items = 0;
-    HIPCHK(hipMemcpyAsync(&tot, c->total.p, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(&items, c->item_count.p, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (items != c->item_total) return fail(NQ_ECAPACITY, "work items changed between sizing and run");
-    c->last_nodes = count_nodes != 0;
+    NQ_HIPCHK(hipMemcpyAsync(&tot, c->total.p, 8, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipMemcpyAsync(&items, c->item_count.p, 8, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    if (items != c->item_total) return nq_fail(NQ_ECAPACITY, "work items changed between sizing and run");
     if (total_out) *total_out = tot;
     if (t) {
         if (int r = elapsed(c, 1, 2, &t->kernel_ms)) return r;
@@ -388,12 +325,12 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
 int nq_ctx_results(nq_ctx *c, uint64_t *weighted_out, uint64_t *nodes_out)
 {
     if (int r = check_ctx(c)) return r;
-    if (!c->loaded) return fail(NQ_ESTATE, "no task set loaded");
+    if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
     if (c->m) {
-        if (weighted_out) HIPCHK(hipMemcpyAsync(weighted_out, c->weighted.p, c->m * 8, hipMemcpyDeviceToHost, c->stream));
-        if (nodes_out) HIPCHK(hipMemcpyAsync(nodes_out, c->nodes.p, c->m * 8, hipMemcpyDeviceToHost, c->stream));
+        if (weighted_out) NQ_HIPCHK(hipMemcpyAsync(weighted_out, c->weighted.p, c->m * 8, hipMemcpyDeviceToHost, c->stream));
+        if (nodes_out) NQ_HIPCHK(hipMemcpyAsync(nodes_out, c->nodes.p, c->m * 8, hipMemcpyDeviceToHost, c->stream));
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -411,12 +348,12 @@ int nq_count(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const nq_sched 
 int nq_symmetric_counts(nq_ctx *c, int N, uint64_t *f90_out, uint64_t *f180_out)
 {
     if (int r = check_ctx(c)) return r;
-    if (N < 2 || N > 25) return fail(NQ_EINVAL, "N=%d outside 2..25", N);
-    HIPCHK(nqk::launch_fill_u64(c->stream, c->sym.p, 2, 0));
-    HIPCHK(nqk::launch_symmetric(c->stream, N, c->sym.p));
+    if (N < 2 || N > 25) return nq_fail(NQ_EINVAL, "N=%d outside 2..25", N);
+    NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->sym.p, 2, 0));
+    NQ_HIPCHK(nqk::launch_symmetric(c->stream, N, c->sym.p));
     unsigned long long v[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(v, c->sym.p, 16, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    NQ_HIPCHK(hipMemcpyAsync(v, c->sym.p, 16, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
     if (f90_out) *f90_out = v[0];
     if (f180_out) *f180_out = v[1];
     return 0;
@@ -424,8 +361,8 @@ int nq_symmetric_counts(nq_ctx *c, int N, uint64_t *f90_out, uint64_t *f180_out)
 
 int nq_decode_host(int N, const uint32_t *rec4, uint64_t m, int32_t *out10)
 {
-    if (N < nqt::kMinN || N > nqt::kMaxN) return fail(NQ_EINVAL, "N=%d outside 6..21", N);
-    if (m && (!rec4 || !out10)) return fail(NQ_EINVAL, "null pointer");
+    if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside 6..21", N);
+    if (m && (!rec4 || !out10)) return nq_fail(NQ_EINVAL, "null pointer");
     for (uint64_t a = 0; a < m; a++) {
         nqt::Desc D;
         const uint32_t *r = rec4 + 4 * a;

@@ -1,10 +1,379 @@
-// nq_mlp.hip -- cost-model kernels (placeholder until the MLP lands)
+// nq_mlp.hip -- the learned cost model: per-constellation features and a
+// 3-layer FP32 MLP (24 -> 64 -> 64 -> 1, ReLU) with its forward, backward
+// and Adam kernels.  The model predicts log(1 + DFS nodes) of a
+// constellation; it only orders work, never changes a count.
+//
+// Parameter layout (NQ_MLP_NPARAM floats):
+//   W1[64][24] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "../../include/nq.h"
-extern "C" {
-int nq_ctx_features(nq_ctx *, float *) { return NQ_EUNSUPPORTED; }
-int nq_mlp_forward(nq_ctx *, const float *, const float *, uint64_t, float *) { return NQ_EUNSUPPORTED; }
-int nq_mlp_loss_grad(nq_ctx *, const float *, const float *, const float *, uint64_t, float *, float *) { return NQ_EUNSUPPORTED; }
-int nq_mlp_train(nq_ctx *, float *, const float *, const float *, uint64_t, const uint32_t *, int, int, float, float *) { return NQ_EUNSUPPORTED; }
+#include <math.h>
+#include <algorithm>
+#include "nq_task.h"
+#include "nq_kernels.h"
+#include "nq_dfs.h"
+
+namespace nqk {
+
+constexpr int kIn = 24, kH = 64;
+constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
+              kB3 = kW3 + kH, kNP = kB3 + 1;
+
+// ---------------------------------------------------------------------------
+// features: 24 floats per record, from the record and its decoded descriptor
+// ---------------------------------------------------------------------------
+__global__ void k_features(int N, const uint4 *__restrict__ rec, const uint4 *__restrict__ root,
+                           const uint4 *__restrict__ sched, uint32_t m, float *__restrict__ feat)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    const uint4 r = rec[c];
+    nqt::Desc D;
+    nqt::decode(N, r.x, r.y, r.z, r.w, D);
+    const uint4 fr = root[c];
+    const uint32_t mask = (1u << N) - 1u;
+    const float inv = 1.0f / (float)N;
+    float *f = feat + (size_t)c * kIn;
+    f[0] = D.i * inv;
+    f[1] = D.j * inv;
+    f[2] = D.k * inv;
+    f[3] = D.l * inv;
+    f[4] = D.start * inv;
+    f[5] = D.fid * (1.0f / 27.0f);
+    f[6] = __builtin_popcount(D.free0) * inv;
+    f[7] = (D.endmark - D.start) * inv;
+    f[8] = (D.mark1 > D.start ? D.mark1 - D.start : 0) * inv;
+    f[9] = (D.mark2 > D.start ? D.mark2 - D.start : 0) * inv;
+    f[10] = (D.jmark > D.start ? D.jmark - D.start : 0) * inv;
+    f[11] = (float)D.wshift * (1.0f / 3.0f);
+    f[12] = __builtin_popcount(fr.x & mask) * inv;
+    f[13] = __builtin_popcount(fr.y & mask) * inv;
+    f[14] = __builtin_popcount(fr.z & mask) * inv;
+    // free squares of the next five rows under the current diagonals/columns
+    float lsum = 0.f;
+    for (int t = 1; t <= 5; t++) {
+        const uint32_t nf = mask & ~((fr.x << t) | (fr.y >> t) | fr.z);
+        const int pc = __builtin_popcount(nf);
+        f[14 + t] = pc * inv;
+        lsum += log2f(1.0f + (float)pc);
+    }
+    f[20] = (float)D.T * inv;
+    f[21] = (float)N * (1.0f / 21.0f);
+    int marks = 0;
+    for (int d = 0; d < (int)D.T; d++) marks += ((nqt::param_byte(D.tab, d) & 3u) != 0u);
+    f[22] = (float)marks * 0.5f;
+    f[23] = lsum * 0.1f;
 }
+
+// ---------------------------------------------------------------------------
+// forward: one thread per row; parameters staged in LDS
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stage_params(const float *__restrict__ params, float *sp)
+{
+    for (int a = threadIdx.x; a < kNP; a += blockDim.x) sp[a] = params[a];
+    __syncthreads();
+}
+
+// rows == nullptr: row b is x[b]; else x[rows[b]].  h1/h2 (may be null)
+// receive the post-ReLU activations of the m rows for the backward pass.
+__global__ void k_mlp_forward(const float *__restrict__ params, const float *__restrict__ x,
+                              const uint32_t *__restrict__ rows, uint32_t m, float *__restrict__ y,
+                              float *__restrict__ h1s, float *__restrict__ h2s)
+{
+    __shared__ float sp[kNP];
+    stage_params(params, sp);
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m) return;
+    const uint32_t src = rows ? rows[b] : b;
+    float xin[kIn];
+#pragma unroll
+    for (int i = 0; i < kIn; i++) xin[i] = x[(size_t)src * kIn + i];
+    float h1[kH];
+#pragma unroll 4
+    for (int o = 0; o < kH; o++) {
+        float z = sp[kB1 + o];
+#pragma unroll
+        for (int i = 0; i < kIn; i++) z = fmaf(sp[kW1 + o * kIn + i], xin[i], z);
+        h1[o] = fmaxf(z, 0.f);
+    }
+    float out = sp[kB3];
+    for (int o = 0; o < kH; o++) {
+        float z = sp[kB2 + o];
+#pragma unroll 8
+        for (int i = 0; i < kH; i++) z = fmaf(sp[kW2 + o * kH + i], h1[i], z);
+        const float h = fmaxf(z, 0.f);
+        if (h2s) h2s[(size_t)b * kH + o] = h;
+        out = fmaf(sp[kW3 + o], h, out);
+    }
+    if (h1s) {
+#pragma unroll 4
+        for (int o = 0; o < kH; o++) h1s[(size_t)b * kH + o] = h1[o];
+    }
+    y[b] = out;
+}
+
+// per-row backward of the loss mean((y - t)^2): dz3, dz2 and dz1 (pre-ReLU grads)
+__global__ void k_mlp_backward_rows(const float *__restrict__ params, const float *__restrict__ target,
+                                    const uint32_t *__restrict__ rows, uint32_t m, float inv_m,
+                                    const float *__restrict__ y, const float *__restrict__ h1s,
+                                    const float *__restrict__ h2s, float *__restrict__ dz1s,
+                                    float *__restrict__ dz2s, float *__restrict__ dz3s, float *__restrict__ loss)
+{
+    __shared__ float sp[kNP];
+    __shared__ float red[256];
+    stage_params(params, sp);
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (b < m) {
+        const uint32_t src = rows ? rows[b] : b;
+        const float e = y[b] - target[src];
+        l = e * e;
+        const float dz3 = 2.f * e * inv_m;
+        dz3s[b] = dz3;
+        float dz2[kH];
+#pragma unroll 4
+        for (int o = 0; o < kH; o++) {
+            const float h = h2s[(size_t)b * kH + o];
+            dz2[o] = h > 0.f ? dz3 * sp[kW3 + o] : 0.f;
+            dz2s[(size_t)b * kH + o] = dz2[o];
+        }
+        for (int i = 0; i < kH; i++) {
+            const float h = h1s[(size_t)b * kH + i];
+            float g = 0.f;
+            if (h > 0.f) {
+#pragma unroll 8
+                for (int o = 0; o < kH; o++) g = fmaf(sp[kW2 + o * kH + i], dz2[o], g);
+            }
+            dz1s[(size_t)b * kH + i] = g;
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_m);
+}
+
+// weight gradients: one thread per parameter, summing over the m rows
+__global__ void k_mlp_wgrad(const float *__restrict__ x, const uint32_t *__restrict__ rows, uint32_t m,
+                            const float *__restrict__ h1s, const float *__restrict__ h2s,
+                            const float *__restrict__ dz1s, const float *__restrict__ dz2s,
+                            const float *__restrict__ dz3s, float *__restrict__ grad)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kNP) return;
+    float g = 0.f;
+    if (p < kB1) {                       // W1[o][i]
+        const int o = p / kIn, i = p % kIn;
+        for (uint32_t b = 0; b < m; b++) {
+            const uint32_t src = rows ? rows[b] : b;
+            g = fmaf(dz1s[(size_t)b * kH + o], x[(size_t)src * kIn + i], g);
+        }
+    } else if (p < kW2) {                // b1[o]
+        const int o = p - kB1;
+        for (uint32_t b = 0; b < m; b++) g += dz1s[(size_t)b * kH + o];
+    } else if (p < kB2) {                // W2[o][i]
+        const int q = p - kW2, o = q / kH, i = q % kH;
+        for (uint32_t b = 0; b < m; b++) g = fmaf(dz2s[(size_t)b * kH + o], h1s[(size_t)b * kH + i], g);
+    } else if (p < kW3) {                // b2[o]
+        const int o = p - kB2;
+        for (uint32_t b = 0; b < m; b++) g += dz2s[(size_t)b * kH + o];
+    } else if (p < kB3) {                // W3[i]
+        const int i = p - kW3;
+        for (uint32_t b = 0; b < m; b++) g = fmaf(dz3s[b], h2s[(size_t)b * kH + i], g);
+    } else {                             // b3
+        for (uint32_t b = 0; b < m; b++) g += dz3s[b];
+    }
+    grad[p] = g;
+}
+
+__global__ void k_adam(float *__restrict__ params, const float *__restrict__ grad, float *__restrict__ m1,
+                       float *__restrict__ m2, uint32_t n, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2, float scale)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float g = grad[p] * scale;
+    const float a = b1 * m1[p] + (1.f - b1) * g;
+    const float v = b2 * m2[p] + (1.f - b2) * g * g;
+    m1[p] = a;
+    m2[p] = v;
+    params[p] -= lr * (a / bc1) / (sqrtf(v / bc2) + eps);
+}
+
+hipError_t launch_features(hipStream_t s, int N, const uint4 *rec, const uint4 *root, const uint4 *sched,
+                           uint32_t m, float *feat)
+{
+    if (m == 0) return hipSuccess;
+    (void)sched;
+    k_features<<<(m + 255) / 256, 256, 0, s>>>(N, rec, root, sched, m, feat);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x, uint32_t m, float *y,
+                              float *h1, float *h2)
+{
+    if (m == 0) return hipSuccess;
+    k_mlp_forward<<<(m + 255) / 256, 256, 0, s>>>(params, x, nullptr, m, y, h1, h2);
+    return hipGetLastError();
+}
+
+}  // namespace nqk
+
+// ---------------------------------------------------------------------------
+// host side of the cost-model entry points
+// ---------------------------------------------------------------------------
+#include "nq_ctx.h"
+
+namespace {
+
+struct MlpBufs {
+    nqk_buf::Buf<float> params, grad, m1, m2, x, y, pred, h1, h2, dz1, dz2, dz3, loss;
+    nqk_buf::Buf<uint32_t> rows;
+};
+
+}  // namespace
+
+extern "C" {
+
+int nq_ctx_features(nq_ctx *c, float *feat_out)
+{
+    NQ_CTX_CHECK(c);
+    if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
+    if (!feat_out) return nq_fail(NQ_EINVAL, "null output");
+    const uint64_t m = c->m;
+    if (m == 0) return 0;
+    nqk_buf::Buf<float> f;
+    NQ_HIPCHK(f.reserve(m * nqk::kIn));
+    NQ_HIPCHK(nqk::launch_features(c->stream, c->N, c->rec.p, c->root.p, c->sched.p,
+                                   (uint32_t)m, f.p));
+    NQ_HIPCHK(hipMemcpyAsync(feat_out, f.p, m * nqk::kIn * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int nq_mlp_forward(nq_ctx *c, const float *params, const float *x, uint64_t m, float *y_out)
+{
+    NQ_CTX_CHECK(c);
+    if (!params || (m && (!x || !y_out))) return nq_fail(NQ_EINVAL, "null pointer");
+    if (m >= (1ull << 31)) return nq_fail(NQ_EINVAL, "too many rows");
+    if (m == 0) return 0;
+    hipStream_t s = c->stream;
+    nqk_buf::Buf<float> P, X, Y;
+    NQ_HIPCHK(P.reserve(nqk::kNP));
+    NQ_HIPCHK(X.reserve(m * nqk::kIn));
+    NQ_HIPCHK(Y.reserve(m));
+    NQ_HIPCHK(hipMemcpyAsync(P.p, params, nqk::kNP * sizeof(float), hipMemcpyHostToDevice, s));
+    NQ_HIPCHK(hipMemcpyAsync(X.p, x, m * nqk::kIn * sizeof(float), hipMemcpyHostToDevice, s));
+    NQ_HIPCHK(nqk::launch_mlp_forward(s, P.p, X.p, (uint32_t)m, Y.p, nullptr, nullptr));
+    NQ_HIPCHK(hipMemcpyAsync(y_out, Y.p, m * sizeof(float), hipMemcpyDeviceToHost, s));
+    NQ_HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+namespace {
+
+// one minibatch: forward, per-row backward, weight gradient (into B.grad), loss added to B.loss
+int batch_grad(hipStream_t s, MlpBufs &B, const uint32_t *rows, uint32_t nb, float inv_m)
+{
+    using namespace nqk;
+    const uint32_t g = (nb + 255) / 256;
+    k_mlp_forward<<<g, 256, 0, s>>>(B.params.p, B.x.p, rows, nb, B.pred.p, B.h1.p, B.h2.p);
+    NQ_HIPCHK(hipGetLastError());
+    k_mlp_backward_rows<<<g, 256, 0, s>>>(B.params.p, B.y.p, rows, nb, inv_m, B.pred.p, B.h1.p, B.h2.p,
+                                          B.dz1.p, B.dz2.p, B.dz3.p, B.loss.p);
+    NQ_HIPCHK(hipGetLastError());
+    k_mlp_wgrad<<<(kNP + 255) / 256, 256, 0, s>>>(B.x.p, rows, nb, B.h1.p, B.h2.p, B.dz1.p, B.dz2.p, B.dz3.p,
+                                                  B.grad.p);
+    NQ_HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int stage_xy(hipStream_t s, MlpBufs &B, const float *params, const float *x, const float *y, uint64_t m,
+             uint32_t rows_cap)
+{
+    NQ_HIPCHK(B.params.reserve(nqk::kNP));
+    NQ_HIPCHK(B.grad.reserve(nqk::kNP));
+    NQ_HIPCHK(B.x.reserve(m * nqk::kIn));
+    NQ_HIPCHK(B.y.reserve(m));
+    NQ_HIPCHK(B.pred.reserve(rows_cap));
+    NQ_HIPCHK(B.h1.reserve((size_t)rows_cap * nqk::kH));
+    NQ_HIPCHK(B.h2.reserve((size_t)rows_cap * nqk::kH));
+    NQ_HIPCHK(B.dz1.reserve((size_t)rows_cap * nqk::kH));
+    NQ_HIPCHK(B.dz2.reserve((size_t)rows_cap * nqk::kH));
+    NQ_HIPCHK(B.dz3.reserve(rows_cap));
+    NQ_HIPCHK(B.loss.reserve(1));
+    NQ_HIPCHK(hipMemcpyAsync(B.params.p, params, nqk::kNP * sizeof(float), hipMemcpyHostToDevice, s));
+    NQ_HIPCHK(hipMemcpyAsync(B.x.p, x, m * nqk::kIn * sizeof(float), hipMemcpyHostToDevice, s));
+    NQ_HIPCHK(hipMemcpyAsync(B.y.p, y, m * sizeof(float), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+}  // namespace
+
+int nq_mlp_loss_grad(nq_ctx *c, const float *params, const float *x, const float *y, uint64_t m, float *loss_out,
+                     float *grad_out)
+{
+    NQ_CTX_CHECK(c);
+    if (!params || !loss_out || !grad_out || (m && (!x || !y))) return nq_fail(NQ_EINVAL, "null pointer");
+    if (m == 0 || m >= (1ull << 24)) return nq_fail(NQ_EINVAL, "rows must be in 1..2^24");
+    hipStream_t s = c->stream;
+    MlpBufs B;
+    if (int r = stage_xy(s, B, params, x, y, m, (uint32_t)m)) return r;
+    NQ_HIPCHK(hipMemsetAsync(B.loss.p, 0, sizeof(float), s));
+    if (int r = batch_grad(s, B, nullptr, (uint32_t)m, 1.0f / (float)m)) return r;
+    NQ_HIPCHK(hipMemcpyAsync(grad_out, B.grad.p, nqk::kNP * sizeof(float), hipMemcpyDeviceToHost, s));
+    NQ_HIPCHK(hipMemcpyAsync(loss_out, B.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    NQ_HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int nq_mlp_train(nq_ctx *c, float *params, const float *x, const float *y, uint64_t m, const uint32_t *perm,
+                 int epochs, int batch, float lr, float *loss_out)
+{
+    NQ_CTX_CHECK(c);
+    if (!params || !x || !y || !perm || !loss_out) return nq_fail(NQ_EINVAL, "null pointer");
+    if (m == 0 || m >= (1ull << 24)) return nq_fail(NQ_EINVAL, "rows must be in 1..2^24");
+    if (epochs < 1 || batch < 1 || batch > 65536) return nq_fail(NQ_EINVAL, "bad epochs/batch");
+    for (uint64_t a = 0; a < m * (uint64_t)epochs; a++)
+        if (perm[a] >= m) return nq_fail(NQ_EINVAL, "perm entry out of range");
+    hipStream_t s = c->stream;
+    MlpBufs B;
+    if (int r = stage_xy(s, B, params, x, y, m, (uint32_t)batch)) return r;
+    NQ_HIPCHK(B.m1.reserve(nqk::kNP));
+    NQ_HIPCHK(B.m2.reserve(nqk::kNP));
+    NQ_HIPCHK(B.rows.reserve(m * (uint64_t)epochs));
+    NQ_HIPCHK(hipMemsetAsync(B.m1.p, 0, nqk::kNP * sizeof(float), s));
+    NQ_HIPCHK(hipMemsetAsync(B.m2.p, 0, nqk::kNP * sizeof(float), s));
+    NQ_HIPCHK(hipMemcpyAsync(B.rows.p, perm, m * (uint64_t)epochs * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    double p1 = 1.0, p2 = 1.0;
+    for (int e = 0; e < epochs; e++) {
+        NQ_HIPCHK(hipMemsetAsync(B.loss.p, 0, sizeof(float), s));
+        const uint32_t *rows = B.rows.p + (uint64_t)e * m;
+        uint32_t nbatch = 0;
+        for (uint64_t off = 0; off < m; off += (uint64_t)batch) {
+            const uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)batch, m - off);
+            if (int r = batch_grad(s, B, rows + off, nb, 1.0f / (float)nb)) return r;
+            p1 *= b1;
+            p2 *= b2;
+            nqk::k_adam<<<(nqk::kNP + 255) / 256, 256, 0, s>>>(B.params.p, B.grad.p, B.m1.p, B.m2.p, nqk::kNP, lr,
+                                                              b1, b2, eps, (float)(1.0 - p1), (float)(1.0 - p2),
+                                                              1.0f);
+            NQ_HIPCHK(hipGetLastError());
+            nbatch++;
+        }
+        float l = 0.f;
+        NQ_HIPCHK(hipMemcpyAsync(&l, B.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+        NQ_HIPCHK(hipStreamSynchronize(s));
+        loss_out[e] = l / (float)nbatch;
+    }
+    NQ_HIPCHK(hipMemcpyAsync(params, B.params.p, nqk::kNP * sizeof(float), hipMemcpyDeviceToHost, s));
+    NQ_HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"
