@@ -135,6 +135,17 @@ def main() -> int:
     plain_value = nodes_total / plain_wall
     log(f"[bench] plain {1000 * plain_wall:.2f} ms/count, {plain_value / 1e9:.2f} G nodes/s")
 
+    # DFS-kernel time for the roofline: the same runs outside the graph,
+    # bracketed by events around the persistent kernel alone
+    if sched.use_graph:
+        eager = nqueens.Sched(**{f: getattr(sched, f) for f, _ in sched._fields_})
+        eager.use_graph = 0
+        ctx.load(N, rec, order)
+        kernel_ms = []
+        for _ in range(max(2, args.steps)):
+            tot, t = ctx.run_batched(eager)
+            assert tot == expect_total
+            kernel_ms.append(t.kernel_ms)
     kavg = float(np.mean(kernel_ms))
     achieved = nodes_total * OPS_PER_NODE / (kavg / 1000.0) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(PEAK_INT32_TOPS, 2),

@@ -56,7 +56,8 @@ typedef struct {
 } nq_sched;
 
 typedef struct {
-    double kernel_ms;      /* device time of the DFS kernel(s), HIP events on the context stream */
+    double kernel_ms;      /* device time of the DFS kernel, HIP events on the context stream
+                              (-1 for a graph replay, which is timed as a whole) */
     double total_ms;       /* device time of the whole run (expand + DFS + finalize) */
     uint64_t items;        /* work items after splitting */
     uint64_t launches;     /* DFS kernel launches in the run */

@@ -286,7 +286,6 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         per_cu = std::min(per_cu, want);
     }
     const int grid = c->cu * per_cu;
-    bool timed = true;
     if (s.use_graph) {
         nq_graph_key k;
         k.split = s.split_depth; k.block = s.block; k.lds = s.lds_stack; k.chunk = s.chunk; k.grid = grid;
@@ -294,7 +293,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         if (!c->gexec || !(k == c->gkey)) {
             drop_graph(c);
             NQ_HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int r = enqueue_batched(c, s, count_nodes, grid, true);
+            int r = enqueue_batched(c, s, count_nodes, grid, false);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (r) { if (g) (void)hipGraphDestroy(g); return r; }
@@ -303,9 +302,12 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
             NQ_HIPCHK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
             c->gkey = k;
         }
+        // events bracket the whole replay; the DFS kernel alone is not timed in graph mode
+        NQ_HIPCHK(hipEventRecord(c->ev[0], c->stream));
         NQ_HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        NQ_HIPCHK(hipEventRecord(c->ev[3], c->stream));
     } else {
-        if (int r = enqueue_batched(c, s, count_nodes, grid, timed)) return r;
+        if (int r = enqueue_batched(c, s, count_nodes, grid, true)) return r;
     }
     unsigned long long tot = 0, items = 0;
     NQ_HIPCHK(hipMemcpyAsync(&tot, c->total.p, 8, hipMemcpyDeviceToHost, c->stream));
@@ -314,7 +316,8 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     if (items != c->item_total) return nq_fail(NQ_ECAPACITY, "work items changed between sizing and run");
     if (total_out) *total_out = tot;
     if (t) {
-        if (int r = elapsed(c, 1, 2, &t->kernel_ms)) return r;
+        if (s.use_graph) t->kernel_ms = -1.0;
+        else if (int r = elapsed(c, 1, 2, &t->kernel_ms)) return r;
         if (int r = elapsed(c, 0, 3, &t->total_ms)) return r;
         t->items = items;
         t->launches = 1;

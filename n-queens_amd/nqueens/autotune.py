@@ -90,3 +90,44 @@ def record_best(N: int, preset: int, rows: list[dict], path: Path = TUNED_PATH) 
     TUNED.clear()
     TUNED.update(table)
     return best
+
+
+def main(argv=None) -> None:
+    """python -m nqueens.autotune --n 19 --preset 5 [--out PATH]: sweep, print
+    one JSON row per configuration, store the best in tuned.json (and PATH)."""
+    import argparse
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=19)
+    ap.add_argument("--preset", type=int, default=5)
+    ap.add_argument("--splits", default="2,3,4,5,6")
+    ap.add_argument("--blocks", default="64,128,256")
+    ap.add_argument("--lds", default="1,0")
+    ap.add_argument("--chunks", default="64")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="fifo")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    from .constellations import gen_constellations
+    from .solver import default_context
+    known = json.loads((Path(__file__).resolve().parents[2] / "tests" / "golden" / "known.json").read_text())
+    ctx = default_context()
+    rec = gen_constellations(a.n, a.preset)
+    order = None
+    if a.scheduler == "mlp":
+        from .costmodel import default_model
+        order = default_model().order(a.n, rec, ctx=ctx)
+    ctx.load(a.n, rec, order)
+    ints = lambda s: tuple(int(x) for x in s.split(","))
+    rows = sweep(ctx, a.n, known["total_a000170"][str(a.n)], ints(a.splits), ints(a.blocks), ints(a.lds),
+                 ints(a.chunks), reps=a.reps, log=lambda r: print(json.dumps(r), flush=True))
+    best = record_best(a.n, a.preset, rows)
+    print(json.dumps({"best": best, "n": a.n, "preset": a.preset, "scheduler": a.scheduler}), flush=True)
+    if a.out:
+        Path(a.out).write_text(json.dumps({"rows": rows, "best": best, "n": a.n, "preset": a.preset,
+                                           "scheduler": a.scheduler}, indent=1) + "\n")
+    sys.stdout.flush()
+
+
+if __name__ == "__main__":
+    main()

@@ -138,18 +138,26 @@ def default_model() -> CostModel:
     return _default
 
 
-def main() -> None:
-    """Collect labels at N=16..19 (preset 5), train, report, save."""
+def main(argv=None) -> None:
+    """Collect labels at N=16..19 (preset 5), train, report, save
+    (python -m nqueens.costmodel [--out PATH] to also write a copy)."""
+    import argparse
     import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--epochs", type=int, default=40)
+    a = ap.parse_args(argv)
     ctx = Context(0)
     t0 = time.perf_counter()
     X, y, ns = collect(ctx)
     t1 = time.perf_counter()
-    model, rep = train(ctx, X, y, trained_on=[16, 17, 18, 19])
+    model, rep = train(ctx, X, y, epochs=a.epochs, trained_on=[16, 17, 18, 19])
     t2 = time.perf_counter()
     rep.update({"collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "rows": int(X.shape[0])})
     model.save()
-    print(json.dumps(rep))
+    if a.out:
+        model.save(Path(a.out))
+    print(json.dumps(rep), flush=True)
 
 
 if __name__ == "__main__":
