@@ -66,7 +66,7 @@ struct nq_ctx {
     // resident task set
     int N = 0;
     uint64_t m = 0;
-    uint32_t Tmax = 0;
+    uint32_t Tmax = 0, Tmin = 0;
     bool loaded = false;
     nqk_buf::Buf<uint4> rec, root, sched, items;
     nqk_buf::Buf<uint32_t> order, nitems, off, item_c, err, head;

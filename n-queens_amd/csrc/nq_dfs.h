@@ -51,28 +51,26 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 
 // Solutions below a pre-terminal frame (depth T-1) without expanding it.
 // For a placement `bit` the terminal free set is F & ~(bit | bit<<s | bit>>s),
-// F = free set of the terminal row before that placement.  A bit is a dead
-// end exactly when F lies inside those (at most three) squares, so the dead
-// bits are the intersection of spread(f) over the bits f of F, empty when F
-// has four or more bits.
+// F = free set of the terminal row before that placement.  A bit b is a
+// dead end exactly when every f in F is one of b, b<<s, b>>s, i.e. when b is
+// covered by all |F| of the masks F<<s, F, F>>s at once.  The three masks
+// cover a position at most once per f, so the dead bits are: any cover for
+// |F| = 1, the majority of the three masks for |F| = 2, all three for
+// |F| = 3, none for |F| >= 4, and every bit when F is empty.
 NQ_HD inline uint32_t term_count(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
-                                               uint32_t p, uint32_t tm, uint32_t nsh)
+                                 uint32_t p, uint32_t tm, uint32_t nsh)
 {
     const uint32_t s = (p & 3u) + 1u;
     const uint32_t X = (ld << s) | ((p >> 2) & 3u) | (rd >> s) | (((p >> 4) & 3u) << nsh) | col;
     const uint32_t F = tm & ~X;
-    const uint32_t f1 = F & (0u - F);
-    const uint32_t F2 = F ^ f1;
-    const uint32_t f2 = F2 & (0u - F2);
-    const uint32_t F3 = F2 ^ f2;
-    const uint32_t f3 = F3 & (0u - F3);
-    const uint32_t F4 = F3 ^ f3;
-    uint32_t B = f1 | (f1 >> s) | (f1 << s);
-    B &= f2 ? (f2 | (f2 >> s) | (f2 << s)) : ~0u;
-    B &= f3 ? (f3 | (f3 >> s) | (f3 << s)) : ~0u;
-    B = F4 ? 0u : B;
-    B = F ? B : ~0u;
-    return __builtin_popcount(avail & ~B);
+    const uint32_t a = F << s, c = F >> s;
+    const uint32_t pc = (uint32_t)__builtin_popcount(F);
+    const uint32_t any = a | F | c;
+    const uint32_t maj = (a & F) | (a & c) | (F & c);
+    const uint32_t all = a & F & c;
+    uint32_t B = (pc == 1u) ? any : ((pc == 2u) ? maj : ((pc == 3u) ? all : 0u));
+    B = (pc == 0u) ? ~0u : B;
+    return (uint32_t)__builtin_popcount(avail & ~B);
 }
 
 // ---------------------------------------------------------------------------
@@ -148,6 +146,62 @@ NQ_HD inline void dfs_subtree(uint32_t mask, uint32_t nsh, const Sched &S, uint3
             p = e.w >> 24;
         }
     }
+}
+
+// A lane's DFS position: the current frame and its stack depth.
+struct Walk {
+    uint32_t ld, rd, col, avail, d, p;
+    int sp;
+};
+
+// One DFS node, without divergent branches: every lane extracts one
+// candidate, resolves a pre-terminal child in place (term_count), and then
+// either descends, keeps its frame, or pops.  The frame remainder is written
+// to the free stack slot `sp` every step (kept only when descending) and the
+// top entry `sp-1` is read every step (used only when popping), so the LDS
+// traffic is uniform and the load is issued a full step ahead of its use.
+// Needs one stack slot more than the deepest push.  Returns true when the
+// walk's subtree is exhausted.
+template <bool NODES, class Stack>
+NQ_HD inline bool walk_step(uint32_t mask, uint32_t nsh, const Sched &S, Walk &w, Stack &st, uint64_t &cnt,
+                            uint64_t &nodes)
+{
+    const int spm1 = w.sp > 0 ? w.sp - 1 : 0;
+    const uint4 top = st.get(spm1);
+    const uint32_t p = w.p;
+    const uint32_t s = (p & 3u) + 1u;
+    const uint32_t bit = w.avail & (0u - w.avail);
+    const uint32_t rest = w.avail ^ bit;
+    uint32_t nld = ((w.ld | bit) << s) | ((p >> 2) & 3u);
+    const uint32_t nrd = ((w.rd | bit) >> s) | (((p >> 4) & 3u) << nsh);
+    const uint32_t ncol = w.col | bit;
+    uint32_t nf = mask & ~(nld | nrd | ncol);
+    const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
+    const uint32_t cd = w.d + 1u;
+    const uint32_t cp = pbyte(S, cd);
+    bool ok = (nf != 0u) && (((p & 0x40u) == 0u) || (look != 0u));
+    const bool jm = (cp & 0x80u) != 0u;
+    nf = jm ? (nf & ~1u) : nf;
+    nld = jm ? (nld | 1u) : nld;
+    ok = ok && (nf != 0u);
+    const bool pre = (cd == S.T - 1u);
+    const bool term = ok && pre;
+    const uint32_t tc = term_count(nld, nrd, ncol, nf, cp, S.tm, nsh);
+    cnt += term ? tc : 0u;
+    if (NODES) nodes += 1u + (term ? (uint32_t)__builtin_popcount(nf) : 0u);
+    st.put(w.sp, pack_frame(w.ld, w.rd, w.col, rest, w.d, p));
+    const bool down = ok && !pre;
+    const bool up = !down && rest == 0u;
+    const bool pop = up && w.sp > 0;
+    const bool push = down && rest != 0u;
+    w.ld = down ? nld : (pop ? top.x : w.ld);
+    w.rd = down ? nrd : (pop ? top.y : w.rd);
+    w.col = down ? ncol : (pop ? top.z : w.col);
+    w.avail = down ? nf : (pop ? (top.w & 0x7FFFFu) : rest);
+    w.d = down ? cd : (pop ? ((top.w >> 19) & 31u) : w.d);
+    w.p = down ? cp : (pop ? (top.w >> 24) : p);
+    w.sp += (push ? 1 : 0) - (pop ? 1 : 0);
+    return up && !pop;
 }
 
 struct NoEmit {

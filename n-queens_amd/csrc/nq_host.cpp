@@ -51,9 +51,15 @@ void drop_graph(nq_ctx *c)
     c->gkey = nq_graph_key{};
 }
 
-// LDS entries per lane: frames with remaining candidates sit at depths
-// below T-1, so T-1 entries always suffice.
-uint32_t stack_entries(const nq_ctx *c) { return c->Tmax > 1 ? c->Tmax - 1 : 1; }
+// Stack slots per lane for a walk from depth d0: frames with remaining
+// candidates sit at depths d0..T-3 (at most T-2-d0 entries) and the walk
+// writes one slot past the top, so T-1-d0 slots suffice.  Items start at
+// depth `split` unless a record's T is too small to split (then at its root).
+uint32_t stack_entries(const nq_ctx *c, int split = 0)
+{
+    uint32_t d0 = (split > 0 && (uint32_t)split + 2u <= c->Tmin) ? (uint32_t)split : 0u;
+    return c->Tmax > 1u + d0 ? c->Tmax - 1u - d0 : 1u;
+}
 
 int elapsed(nq_ctx *c, int a, int b, double *ms)
 {
@@ -86,7 +92,7 @@ int size_items(nq_ctx *c, int split, int block)
 // DFS, finalize.  Events 0/3 bracket the sequence, 1/2 the DFS kernel.
 int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool events)
 {
-    const uint32_t m = (uint32_t)c->m, sd = stack_entries(c);
+    const uint32_t m = (uint32_t)c->m, sd = stack_entries(c), sdb = stack_entries(c, s.split_depth);
     hipStream_t st = c->stream;
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[0], st));
     NQ_HIPCHK(nqk::launch_fill_u64(st, c->raw.p, m, 0));
@@ -100,7 +106,7 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
                               (uint32_t)s.split_depth, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
                               c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
-    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sd, c->N, c->items.p,
+    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sdb, c->N, c->items.p,
                                c->item_c.p, c->sched.p, c->item_count.p, c->head.p, (uint32_t)s.chunk,
                                c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[2], st));
@@ -217,7 +223,11 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
     if (m) {
         std::vector<uint4> sc(2 * m);
         NQ_HIPCHK(hipMemcpy(sc.data(), c->sched.p, 2 * m * 16, hipMemcpyDeviceToHost));
-        for (uint64_t a = 0; a < m; a++) c->Tmax = std::max(c->Tmax, sc[2 * a + 1].y & 0xFFu);
+        c->Tmin = 0xFFu;
+        for (uint64_t a = 0; a < m; a++) {
+            c->Tmax = std::max(c->Tmax, sc[2 * a + 1].y & 0xFFu);
+            c->Tmin = std::min(c->Tmin, sc[2 * a + 1].y & 0xFFu);
+        }
     }
     if (c->Tmax > (uint32_t)nqk::kStackCap) return nq_fail(NQ_ECAPACITY, "terminal depth %u beyond stack %d", c->Tmax, nqk::kStackCap);
     c->loaded = true;
@@ -265,8 +275,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     // frames at depth T-1 are resolved in place, so items live at depth <= T-2
     const int max_split = c->Tmax >= 2 ? (int)c->Tmax - 2 : 0;
     if (s.split_depth > max_split) s.split_depth = max_split;
-    const uint32_t sd = stack_entries(c);
-    const size_t lds_bytes = (size_t)s.block * sd * 16;
+    const size_t lds_bytes = (size_t)s.block * stack_entries(c, s.split_depth) * 16;
     if (lds_bytes > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, c->lds_max);
     if (c->m == 0) {
         if (total_out) *total_out = 0;

@@ -38,7 +38,9 @@ __device__ __forceinline__ void run_root(uint32_t mask, uint32_t nsh, const Sche
         if (NODES) nodes = (uint32_t)__builtin_popcount(root.w);
         return;
     }
-    dfs_subtree<NODES, false>(mask, nsh, S, root.x, root.y, root.z, root.w, 0u, 0u, st, cnt, nodes, NoEmit{});
+    Walk w{root.x, root.y, root.z, root.w, 0u, pbyte(S, 0), 0};
+    while (!walk_step<NODES>(mask, nsh, S, w, st, cnt, nodes)) {
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -173,8 +175,8 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
     uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
     bool exhausted = false;           // wave-uniform: queue drained
     bool have = false;                // this lane holds a subtree
-    uint32_t c = 0, ld = 0, rd = 0, col = 0, avail = 0, d = 0, p = 0;
-    int sp = 0;
+    uint32_t c = 0;
+    Walk w{0u, 0u, 0u, 0u, 0u, 0u, 0};
     Sched S{};
     uint64_t cnt = 0, nn = 0;
     for (;;) {
@@ -195,18 +197,16 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
                     const uint4 it = items[idx];
                     c = item_c[idx];
                     S = load_sched(sched, c);
-                    d = it.w >> 19;
-                    avail = it.w & 0x7FFFFu;
-                    ld = it.x; rd = it.y; col = it.z;
-                    cnt = 0; nn = 0; sp = 0;
-                    if (avail != 0u) {
+                    const uint32_t d0 = it.w >> 19, av = it.w & 0x7FFFFu;
+                    cnt = 0; nn = 0;
+                    if (av != 0u) {
                         if (S.T == 0u) {
-                            cnt = (avail & S.tm) ? 1u : 0u;
-                        } else if (d + 1u == S.T) {
-                            cnt = term_count(ld, rd, col, avail, pbyte(S, d), S.tm, nsh);
-                            if (NODES) nn = (uint32_t)__builtin_popcount(avail);
+                            cnt = (av & S.tm) ? 1u : 0u;
+                        } else if (d0 + 1u == S.T) {
+                            cnt = term_count(it.x, it.y, it.z, av, pbyte(S, d0), S.tm, nsh);
+                            if (NODES) nn = (uint32_t)__builtin_popcount(av);
                         } else {
-                            p = pbyte(S, d);
+                            w = Walk{it.x, it.y, it.z, av, d0, pbyte(S, d0), 0};
                             have = true;
                         }
                     }
@@ -220,48 +220,12 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
         }
         if (exhausted && __ballot(have) == 0ull) break;
         if (!have) continue;
-        // ---- one DFS node (the loop body of dfs_subtree) ----
-        const uint32_t s = (p & 3u) + 1u;
-        const uint32_t bit = avail & (0u - avail);
-        avail ^= bit;
-        uint32_t nld = ((ld | bit) << s) | ((p >> 2) & 3u);
-        const uint32_t nrd = ((rd | bit) >> s) | (((p >> 4) & 3u) << nsh);
-        const uint32_t ncol = col | bit;
-        uint32_t nf = mask & ~(nld | nrd | ncol);
-        const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
-        const uint32_t cd = d + 1u;
-        const uint32_t cp = pbyte(S, cd);
-        bool ok = (nf != 0u) && (((p & 0x40u) == 0u) || (look != 0u));
-        if (cp & 0x80u) { nf &= ~1u; nld |= 1u; }
-        ok = ok && (nf != 0u);
-        if (NODES) nn += 1u;
-        if (cd == S.T - 1u) {
-            if (ok) {
-                cnt += term_count(nld, nrd, ncol, nf, cp, S.tm, nsh);
-                if (NODES) nn += (uint32_t)__builtin_popcount(nf);
-            }
-            ok = false;
-        }
-        if (ok) {
-            if (avail) {
-                const uint4 e = pack_frame(ld, rd, col, avail, d, p);
-                if (LDS_STACK) lst.put(sp, e); else pst.put(sp, e);
-                sp++;
-            }
-            ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; p = cp;
-        } else if (avail == 0u) {
-            if (sp == 0) {
-                if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
-                if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
-                have = false;
-            } else {
-                sp--;
-                const uint4 e = LDS_STACK ? lst.get(sp) : pst.get(sp);
-                ld = e.x; rd = e.y; col = e.z;
-                avail = e.w & 0x7FFFFu;
-                d = (e.w >> 19) & 31u;
-                p = e.w >> 24;
-            }
+        const bool done = LDS_STACK ? walk_step<NODES>(mask, nsh, S, w, lst, cnt, nn)
+                                    : walk_step<NODES>(mask, nsh, S, w, pst, cnt, nn);
+        if (done) {
+            if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
+            if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
+            have = false;
         }
     }
 }

@@ -1,0 +1,46 @@
+"""Fixed workload for profiler runs (rocprofv3 kernel traces and counters).
+
+    python -m nqueens.profile_run --n 19 --preset 5 --mode batched --reps 2
+
+Loads the constellations of (N, preset) once and runs the chosen kernel
+`reps` times outside a graph, so that every dispatch is traced on its own.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+from . import autotune
+from .constellations import gen_constellations
+from .solver import default_context
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=19)
+    ap.add_argument("--preset", type=int, default=5)
+    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--split", type=int, default=None)
+    ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--lds", type=int, default=None)
+    a = ap.parse_args(argv)
+    ctx = default_context()
+    rec = gen_constellations(a.n, a.preset)
+    ctx.load(a.n, rec)
+    s = autotune.best_sched(a.n, a.preset)
+    for k, v in (("split_depth", a.split), ("block", a.block), ("lds_stack", a.lds)):
+        if v is not None:
+            setattr(s, k, v)
+    for r in range(a.reps):
+        if a.mode == "batched":
+            tot, t = ctx.run_batched(s)
+        else:
+            tot, t = ctx.run_plain()
+        print(json.dumps({"rep": r, "mode": a.mode, "n": a.n, "preset": a.preset, "total": tot,
+                          "kernel_ms": t.kernel_ms, "run_ms": t.total_ms, "items": int(t.items),
+                          "split": s.split_depth, "block": s.block, "lds": s.lds_stack}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
