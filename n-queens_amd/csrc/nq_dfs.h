@@ -32,7 +32,11 @@ NQ_HD inline uint32_t pbyte(const Sched &S, uint32_t d)
     const uint32_t a = __builtin_amdgcn_perm(S.t1, S.t0, sel);
     const uint32_t b = __builtin_amdgcn_perm(S.t3, S.t2, sel);
     const uint32_t c = __builtin_amdgcn_perm(0u, S.t4, sel);
-    return (d < 8u) ? a : ((d < 16u) ? b : c);
+    // word-pair choice as masks rather than a conditional, so that no
+    // divergent branch is generated around the three v_perm_b32
+    const uint32_t lo = 0u - (uint32_t)(d < 8u);
+    const uint32_t mid = 0u - (uint32_t)(d < 16u);
+    return (a & lo) | (b & mid & ~lo) | (c & ~mid);
 #else
     const uint32_t w[5] = {S.t0, S.t1, S.t2, S.t3, S.t4};
     return (w[d >> 2] >> ((d & 3u) * 8u)) & 0xFFu;
@@ -74,13 +78,14 @@ NQ_HD inline uint32_t term_count(uint32_t ld, uint32_t rd, uint32_t col, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// DFS stacks: LDS (entry i of lane at base + i*stride) or private memory
+// DFS stacks: LDS (slot i of a lane at base + (i << shift) bytes, i.e. one
+// row of 16-byte entries per slot across the workgroup) or private memory
 // ---------------------------------------------------------------------------
 struct LdsStack {
-    uint4 *base;
-    uint32_t stride;
-    NQ_HD inline void put(int i, uint4 v) { base[i * stride] = v; }
-    NQ_HD inline uint4 get(int i) const { return base[i * stride]; }
+    uint4 *base;       // this lane's slot 0
+    uint32_t shift;    // log2(workgroup size): slot stride in uint4 units
+    NQ_HD inline void put(int i, uint4 v) { base[(uint32_t)i << shift] = v; }
+    NQ_HD inline uint4 get(int i) const { return base[(uint32_t)i << shift]; }
 };
 struct PrivStack {
     uint4 a[kStackCap];
@@ -88,21 +93,27 @@ struct PrivStack {
     NQ_HD inline uint4 get(int i) const { return a[i]; }
 };
 
-// Stack entry: {ld, rd, col, avail | depth<<19 | param<<24}; avail < 2^19.
-NQ_HD inline uint4 pack_frame(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
-                                            uint32_t d, uint32_t p)
+// Frame word: avail (bits 0-18) | depth (19-23) | parameter byte (24-31).
+NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t d, uint32_t p)
 {
-    return make_uint4(ld, rd, col, avail | (d << 19) | (p << 24));
+    return avail | (d << 19) | (p << 24);
+}
+
+// Stack entry: {ld, rd, col, frame word}.
+NQ_HD inline uint4 pack_frame(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
+                              uint32_t d, uint32_t p)
+{
+    return make_uint4(ld, rd, col, frame_word(avail, d, p));
 }
 
 // One DFS from a frame at depth d (d <= T-2, avail != 0 after jmark) until
-// its subtree is exhausted.  SPLIT > 0: children created at depth `split`
-// are handed to emit() instead of being expanded.
+// its subtree is exhausted, handing children created at depth `split` to
+// emit() instead of expanding them.  Used by the work-item expansion.
 template <bool NODES, bool SPLIT, class Stack, class Emit>
 NQ_HD inline void dfs_subtree(uint32_t mask, uint32_t nsh, const Sched &S, uint32_t ld,
-                                            uint32_t rd, uint32_t col, uint32_t avail, uint32_t d,
-                                            uint32_t split, Stack &st, uint64_t &cnt, uint64_t &nodes,
-                                            Emit &&emit)
+                              uint32_t rd, uint32_t col, uint32_t avail, uint32_t d,
+                              uint32_t split, Stack &st, uint64_t &cnt, uint64_t &nodes,
+                              Emit &&emit)
 {
     uint32_t p = pbyte(S, d);
     const uint32_t Tm1 = S.T - 1u;
@@ -148,17 +159,20 @@ NQ_HD inline void dfs_subtree(uint32_t mask, uint32_t nsh, const Sched &S, uint3
     }
 }
 
-// A lane's DFS position: the current frame and its stack depth.
+// A lane's DFS position: the current frame {ld, rd, col, frame word} and
+// the number of stack entries below it.
 struct Walk {
-    uint32_t ld, rd, col, avail, d, p;
+    uint32_t ld, rd, col, W;
     int sp;
 };
 
-// One DFS node, without divergent branches: every lane extracts one
-// candidate, resolves a pre-terminal child in place (term_count), and then
-// either descends, keeps its frame, or pops.  The frame remainder is written
-// to the free stack slot `sp` every step (kept only when descending) and the
-// top entry `sp-1` is read every step (used only when popping), so the LDS
+// One DFS node, without divergent branches.  Every lane extracts one
+// candidate of its frame and builds the child; a child on the terminal row
+// (depth T) counts one solution when its free set meets the terminal mask
+// (the base frame of 18Py:226-234: functionid 14 ignores column bit 0).
+// The lane then descends, keeps its frame, or pops.  The frame remainder is
+// written to the free slot `sp` every step (kept only when descending) and
+// the entry `sp-1` is read every step (used only when popping), so the LDS
 // traffic is uniform and the load is issued a full step ahead of its use.
 // Needs one stack slot more than the deepest push.  Returns true when the
 // walk's subtree is exhausted.
@@ -166,42 +180,42 @@ template <bool NODES, class Stack>
 NQ_HD inline bool walk_step(uint32_t mask, uint32_t nsh, const Sched &S, Walk &w, Stack &st, uint64_t &cnt,
                             uint64_t &nodes)
 {
-    const int spm1 = w.sp > 0 ? w.sp - 1 : 0;
-    const uint4 top = st.get(spm1);
-    const uint32_t p = w.p;
-    const uint32_t s = (p & 3u) + 1u;
-    const uint32_t bit = w.avail & (0u - w.avail);
-    const uint32_t rest = w.avail ^ bit;
-    uint32_t nld = ((w.ld | bit) << s) | ((p >> 2) & 3u);
-    const uint32_t nrd = ((w.rd | bit) >> s) | (((p >> 4) & 3u) << nsh);
+    const uint4 top = st.get(w.sp > 0 ? w.sp - 1 : 0);
+    const uint32_t W = w.W;
+    const uint32_t avail = W & 0x7FFFFu;
+    const uint32_t bit = avail & (0u - avail);
+    const uint32_t rest = W ^ bit;                       // remainder, same depth and parameters
+    const uint32_t s = ((W >> 24) & 3u) + 1u;
+    uint32_t nld = ((w.ld | bit) << s) | ((W >> 26) & 3u);
+    const uint32_t nrd = ((w.rd | bit) >> s) | (((W >> 28) & 3u) << nsh);
     const uint32_t ncol = w.col | bit;
     uint32_t nf = mask & ~(nld | nrd | ncol);
     const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
-    const uint32_t cd = w.d + 1u;
+    const uint32_t cd = ((W >> 19) & 31u) + 1u;
     const uint32_t cp = pbyte(S, cd);
-    bool ok = (nf != 0u) && (((p & 0x40u) == 0u) || (look != 0u));
-    const bool jm = (cp & 0x80u) != 0u;
-    nf = jm ? (nf & ~1u) : nf;
-    nld = jm ? (nld | 1u) : nld;
-    ok = ok && (nf != 0u);
-    const bool pre = (cd == S.T - 1u);
-    const bool term = ok && pre;
-    const uint32_t tc = term_count(nld, nrd, ncol, nf, cp, S.tm, nsh);
-    cnt += term ? tc : 0u;
-    if (NODES) nodes += 1u + (term ? (uint32_t)__builtin_popcount(nf) : 0u);
-    st.put(w.sp, pack_frame(w.ld, w.rd, w.col, rest, w.d, p));
-    const bool down = ok && !pre;
-    const bool up = !down && rest == 0u;
-    const bool pop = up && w.sp > 0;
-    const bool push = down && rest != 0u;
-    w.ld = down ? nld : (pop ? top.x : w.ld);
-    w.rd = down ? nrd : (pop ? top.y : w.rd);
-    w.col = down ? ncol : (pop ? top.z : w.col);
-    w.avail = down ? nf : (pop ? (top.w & 0x7FFFFu) : rest);
-    w.d = down ? cd : (pop ? ((top.w >> 19) & 31u) : w.d);
-    w.p = down ? cp : (pop ? (top.w >> 24) : p);
-    w.sp += (push ? 1 : 0) - (pop ? 1 : 0);
-    return up && !pop;
+    const uint32_t nla = ((W >> 30) & 1u) - 1u;          // all ones when the frame has no lookahead
+    const uint32_t jm = cp >> 7;                         // jmark frame: clear column 0, ld |= 1
+    const bool ok0 = (nf != 0u) & ((look | nla) != 0u);
+    nf &= ~jm;
+    nld |= jm;
+    const bool ok = ok0 & (nf != 0u);
+    const bool leaf = (cd == S.T);
+    cnt += (uint32_t)(leaf & ok & ((nf & S.tm) != 0u));
+    if (NODES) nodes += 1u;
+    st.put(w.sp, make_uint4(w.ld, w.rd, w.col, rest));
+    const bool down = ok & !leaf;
+    const bool more = (rest & 0x7FFFFu) != 0u;
+    const bool pop = !down & !more & (w.sp > 0);
+    const uint32_t Wc = nf | (cd << 19) | (cp << 24);
+    // next frame = child (down), popped entry (pop) or this frame's remainder,
+    // selected with masks so that all lanes run the same instructions
+    const uint32_t dm = 0u - (uint32_t)down, pm = 0u - (uint32_t)pop;
+    w.ld = (nld & dm) | (((top.x & pm) | (w.ld & ~pm)) & ~dm);
+    w.rd = (nrd & dm) | (((top.y & pm) | (w.rd & ~pm)) & ~dm);
+    w.col = (ncol & dm) | (((top.z & pm) | (w.col & ~pm)) & ~dm);
+    w.W = (Wc & dm) | (((top.w & pm) | (rest & ~pm)) & ~dm);
+    w.sp += (int)((uint32_t)(down & more)) - (int)((uint32_t)pop);
+    return !down && !more && !pop;
 }
 
 struct NoEmit {

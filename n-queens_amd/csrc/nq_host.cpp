@@ -52,13 +52,13 @@ void drop_graph(nq_ctx *c)
 }
 
 // Stack slots per lane for a walk from depth d0: frames with remaining
-// candidates sit at depths d0..T-3 (at most T-2-d0 entries) and the walk
-// writes one slot past the top, so T-1-d0 slots suffice.  Items start at
+// candidates sit at depths d0..T-2 (at most T-1-d0 entries) and the walk
+// writes one slot past the top, so T-d0 slots suffice.  Items start at
 // depth `split` unless a record's T is too small to split (then at its root).
 uint32_t stack_entries(const nq_ctx *c, int split = 0)
 {
     uint32_t d0 = (split > 0 && (uint32_t)split + 2u <= c->Tmin) ? (uint32_t)split : 0u;
-    return c->Tmax > 1u + d0 ? c->Tmax - 1u - d0 : 1u;
+    return c->Tmax > d0 ? c->Tmax - d0 : 1u;
 }
 
 int elapsed(nq_ctx *c, int a, int b, double *ms)

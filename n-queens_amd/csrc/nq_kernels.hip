@@ -33,12 +33,7 @@ __device__ __forceinline__ void run_root(uint32_t mask, uint32_t nsh, const Sche
     cnt = 0; nodes = 0;
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
-    if (S.T == 1u) {
-        cnt = term_count(root.x, root.y, root.z, root.w, pbyte(S, 0), S.tm, nsh);
-        if (NODES) nodes = (uint32_t)__builtin_popcount(root.w);
-        return;
-    }
-    Walk w{root.x, root.y, root.z, root.w, 0u, pbyte(S, 0), 0};
+    Walk w{root.x, root.y, root.z, frame_word(root.w, 0u, pbyte(S, 0)), 0};
     while (!walk_step<NODES>(mask, nsh, S, w, st, cnt, nodes)) {
     }
 }
@@ -76,7 +71,7 @@ __global__ void k_plain(int N, const uint4 *__restrict__ root, const uint4 *__re
     if (c < m) {
         const Sched S = load_sched(sched, c);
         const uint32_t wsh = (sched[2 * c + 1].y >> 8) & 3u;
-        LdsStack st{lds + threadIdx.x, blockDim.x};
+        LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
         uint64_t cnt, nn;
         run_root<NODES>(mask, nsh, S, root[c], st, cnt, nn);
         wsum = cnt << wsh;
@@ -107,7 +102,7 @@ __global__ void k_expand(int N, const uint4 *__restrict__ root, const uint4 *__r
     const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
     const Sched S = load_sched(sched, c);
     const uint4 r = root[c];
-    LdsStack st{lds + threadIdx.x, blockDim.x};
+    LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     uint64_t cnt = 0, nn = 0;
     uint32_t n = 0;
     const uint32_t base = WRITE ? off[q] : 0u;
@@ -170,13 +165,13 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
     const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
     const uint32_t n_items = (uint32_t)*nitems_total;
     const uint32_t lane = lane_id();
-    LdsStack lst{lds + threadIdx.x, blockDim.x};
+    LdsStack lst{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     PrivStack pst;
     uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
     bool exhausted = false;           // wave-uniform: queue drained
     bool have = false;                // this lane holds a subtree
     uint32_t c = 0;
-    Walk w{0u, 0u, 0u, 0u, 0u, 0u, 0};
+    Walk w{0u, 0u, 0u, 0u, 0};
     Sched S{};
     uint64_t cnt = 0, nn = 0;
     for (;;) {
@@ -202,11 +197,8 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
                     if (av != 0u) {
                         if (S.T == 0u) {
                             cnt = (av & S.tm) ? 1u : 0u;
-                        } else if (d0 + 1u == S.T) {
-                            cnt = term_count(it.x, it.y, it.z, av, pbyte(S, d0), S.tm, nsh);
-                            if (NODES) nn = (uint32_t)__builtin_popcount(av);
                         } else {
-                            w = Walk{it.x, it.y, it.z, av, d0, pbyte(S, d0), 0};
+                            w = Walk{it.x, it.y, it.z, frame_word(av, d0, pbyte(S, d0)), 0};
                             have = true;
                         }
                     }
