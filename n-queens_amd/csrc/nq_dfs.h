@@ -16,30 +16,28 @@ namespace nqk {
 // per-lane schedule registers
 // ---------------------------------------------------------------------------
 struct Sched {
-    uint32_t t0, t1, t2, t3, t4;  // parameter bytes for depths 0..19
+    uint32_t t0, t1, t2, t3, t4;  // parameter bytes for depths 0..15 (t4 unused by the walk)
     uint32_t T;                   // terminal depth
     uint32_t tm;                  // terminal free mask
 };
 
-// Byte d of the 20-byte table.  v_perm_b32 picks byte (d & 7) of a register
-// pair directly, which keeps the table in VGPRs (an indexed select over the
-// five words is otherwise lowered to a scratch array).  Selector byte 0x0c
-// yields zero, so only byte 0 of each result is non-zero.
+// Byte d of the parameter table, for 0 <= d < 16; zero for d >= 16 (the
+// terminal depth T <= 16 never needs its own byte).  v_perm_b32 picks byte
+// (d & 7) of a register pair directly, which keeps the table in VGPRs (an
+// indexed select over the words is otherwise lowered to a scratch array).
+// Selector byte 0x0c yields zero, so only byte 0 of each result is non-zero;
+// the pair choice uses masks so that no divergent branch is generated.
 NQ_HD inline uint32_t pbyte(const Sched &S, uint32_t d)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const uint32_t sel = 0x0c0c0c00u | (d & 7u);
     const uint32_t a = __builtin_amdgcn_perm(S.t1, S.t0, sel);
     const uint32_t b = __builtin_amdgcn_perm(S.t3, S.t2, sel);
-    const uint32_t c = __builtin_amdgcn_perm(0u, S.t4, sel);
-    // word-pair choice as masks rather than a conditional, so that no
-    // divergent branch is generated around the three v_perm_b32
     const uint32_t lo = 0u - (uint32_t)(d < 8u);
-    const uint32_t mid = 0u - (uint32_t)(d < 16u);
-    return (a & lo) | (b & mid & ~lo) | (c & ~mid);
+    return ((a & lo) | (b & ~lo)) & (0u - (uint32_t)(d < 16u));
 #else
-    const uint32_t w[5] = {S.t0, S.t1, S.t2, S.t3, S.t4};
-    return (w[d >> 2] >> ((d & 3u) * 8u)) & 0xFFu;
+    const uint32_t w[4] = {S.t0, S.t1, S.t2, S.t3};
+    return d < 16u ? (w[d >> 2] >> ((d & 3u) * 8u)) & 0xFFu : 0u;
 #endif
 }
 
@@ -84,13 +82,20 @@ NQ_HD inline uint32_t term_count(uint32_t ld, uint32_t rd, uint32_t col, uint32_
 struct LdsStack {
     uint4 *base;       // this lane's slot 0
     uint32_t shift;    // log2(workgroup size): slot stride in uint4 units
+    NQ_HD inline uint32_t stride() const { return 1u << shift; }
     NQ_HD inline void put(int i, uint4 v) { base[(uint32_t)i << shift] = v; }
     NQ_HD inline uint4 get(int i) const { return base[(uint32_t)i << shift]; }
+    // by offset (slot << shift), as the walk keeps it
+    NQ_HD inline void put_at(uint32_t o, uint4 v) { base[o] = v; }
+    NQ_HD inline uint4 get_at(uint32_t o) const { return base[o]; }
 };
 struct PrivStack {
     uint4 a[kStackCap];
+    NQ_HD inline uint32_t stride() const { return 1u; }
     NQ_HD inline void put(int i, uint4 v) { a[i] = v; }
     NQ_HD inline uint4 get(int i) const { return a[i]; }
+    NQ_HD inline void put_at(uint32_t o, uint4 v) { a[o] = v; }
+    NQ_HD inline uint4 get_at(uint32_t o) const { return a[o]; }
 };
 
 // Frame word: avail (bits 0-18) | depth (19-23) | parameter byte (24-31).
@@ -160,10 +165,10 @@ NQ_HD inline void dfs_subtree(uint32_t mask, uint32_t nsh, const Sched &S, uint3
 }
 
 // A lane's DFS position: the current frame {ld, rd, col, frame word} and
-// the number of stack entries below it.
+// the offset of the first free stack slot (entries below it x slot stride).
 struct Walk {
     uint32_t ld, rd, col, W;
-    int sp;
+    uint32_t so;
 };
 
 // One DFS node, without divergent branches.  Every lane extracts one
@@ -180,7 +185,8 @@ template <bool NODES, class Stack>
 NQ_HD inline bool walk_step(uint32_t mask, uint32_t nsh, const Sched &S, Walk &w, Stack &st, uint64_t &cnt,
                             uint64_t &nodes)
 {
-    const uint4 top = st.get(w.sp > 0 ? w.sp - 1 : 0);
+    const uint32_t stride = st.stride();
+    const uint4 top = st.get_at((uint32_t)max((int)w.so - (int)stride, 0));
     const uint32_t W = w.W;
     const uint32_t avail = W & 0x7FFFFu;
     const uint32_t bit = avail & (0u - avail);
@@ -202,10 +208,10 @@ NQ_HD inline bool walk_step(uint32_t mask, uint32_t nsh, const Sched &S, Walk &w
     const bool leaf = (cd == S.T);
     cnt += (uint32_t)(leaf & ok & ((nf & S.tm) != 0u));
     if (NODES) nodes += 1u;
-    st.put(w.sp, make_uint4(w.ld, w.rd, w.col, rest));
+    st.put_at(w.so, make_uint4(w.ld, w.rd, w.col, rest));
     const bool down = ok & !leaf;
     const bool more = (rest & 0x7FFFFu) != 0u;
-    const bool pop = !down & !more & (w.sp > 0);
+    const bool pop = !down & !more & (w.so != 0u);
     const uint32_t Wc = nf | (cd << 19) | (cp << 24);
     // next frame = child (down), popped entry (pop) or this frame's remainder,
     // selected with masks so that all lanes run the same instructions
@@ -214,7 +220,7 @@ NQ_HD inline bool walk_step(uint32_t mask, uint32_t nsh, const Sched &S, Walk &w
     w.rd = (nrd & dm) | (((top.y & pm) | (w.rd & ~pm)) & ~dm);
     w.col = (ncol & dm) | (((top.z & pm) | (w.col & ~pm)) & ~dm);
     w.W = (Wc & dm) | (((top.w & pm) | (rest & ~pm)) & ~dm);
-    w.sp += (int)((uint32_t)(down & more)) - (int)((uint32_t)pop);
+    w.so += (down & more) ? stride : (pop ? (0u - stride) : 0u);
     return !down && !more && !pop;
 }
 

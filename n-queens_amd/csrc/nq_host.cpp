@@ -229,7 +229,9 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
             c->Tmin = std::min(c->Tmin, sc[2 * a + 1].y & 0xFFu);
         }
     }
-    if (c->Tmax > (uint32_t)nqk::kStackCap) return nq_fail(NQ_ECAPACITY, "terminal depth %u beyond stack %d", c->Tmax, nqk::kStackCap);
+    // the walk keeps 16 parameter bytes per record (depths 0..15), so the
+    // terminal depth T = N-1-preset must not exceed 16
+    if (c->Tmax > 16u) return nq_fail(NQ_EUNSUPPORTED, "terminal depth %u > 16 (preset too small for N)", c->Tmax);
     c->loaded = true;
     return 0;
 }

@@ -33,7 +33,7 @@ __device__ __forceinline__ void run_root(uint32_t mask, uint32_t nsh, const Sche
     cnt = 0; nodes = 0;
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
-    Walk w{root.x, root.y, root.z, frame_word(root.w, 0u, pbyte(S, 0)), 0};
+    Walk w{root.x, root.y, root.z, frame_word(root.w, 0u, pbyte(S, 0)), 0u};
     while (!walk_step<NODES>(mask, nsh, S, w, st, cnt, nodes)) {
     }
 }
@@ -171,7 +171,7 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
     bool exhausted = false;           // wave-uniform: queue drained
     bool have = false;                // this lane holds a subtree
     uint32_t c = 0;
-    Walk w{0u, 0u, 0u, 0u, 0};
+    Walk w{0u, 0u, 0u, 0u, 0u};
     Sched S{};
     uint64_t cnt = 0, nn = 0;
     for (;;) {
@@ -198,7 +198,7 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
                         if (S.T == 0u) {
                             cnt = (av & S.tm) ? 1u : 0u;
                         } else {
-                            w = Walk{it.x, it.y, it.z, frame_word(av, d0, pbyte(S, d0)), 0};
+                            w = Walk{it.x, it.y, it.z, frame_word(av, d0, pbyte(S, d0)), 0u};
                             have = true;
                         }
                     }
