@@ -1,0 +1,51 @@
+"""python -m nqueens [nmin] [nmax] [--mode plain|batched] [--scheduler fifo|mlp] [--preset P]
+
+Prints the reference solver's table (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:2115-2160)
+-- N, Total, Unique, hh:mm:ss.ms and ok/ng against A000170 (18Py:2089) --
+with Unique filled in rather than 0.  Every count runs on the GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+from . import solver
+
+# A000170, as the reference's expected[] (18Py:2089)
+EXPECTED = {6: 4, 7: 40, 8: 92, 9: 352, 10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184,
+            16: 14772512, 17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884, 21: 314666222712}
+
+
+def fmt(sec: float) -> str:
+    ms = int(round(sec * 1000))
+    h, ms = divmod(ms, 3600_000)
+    m, ms = divmod(ms, 60_000)
+    s, ms = divmod(ms, 1000)
+    return f"{h}:{m:02d}:{s:02d}.{ms:03d}"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m nqueens")
+    ap.add_argument("nmin", type=int, nargs="?", default=6)
+    ap.add_argument("nmax", type=int, nargs="?", default=None)
+    ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
+    ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="mlp")
+    ap.add_argument("--preset", type=int, default=None)
+    a = ap.parse_args(argv)
+    nmax = a.nmax if a.nmax is not None else a.nmin
+    ctx = solver.default_context()
+    print(" N:             Total           Unique         hh:mm:ss.ms")
+    bad = 0
+    for N in range(a.nmin, nmax + 1):
+        t0 = time.perf_counter()
+        total, unique = solver.count(N, preset=a.preset, mode=a.mode, scheduler=a.scheduler, ctx=ctx)
+        dt = time.perf_counter() - t0
+        ok = "ok" if EXPECTED.get(N) == total else f"ng({total}!={EXPECTED.get(N)})"
+        bad += ok != "ok"
+        print(f"{N:2d}:{total:18d}{unique:17d}{fmt(dt):>20s}    {ok}", flush=True)
+    return 1 if bad else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

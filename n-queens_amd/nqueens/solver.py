@@ -112,6 +112,9 @@ def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler
         order = default_model().order(N, rec, ctx=ctx)
     elif scheduler != "fifo":
         raise ValueError(f"unknown scheduler {scheduler!r}")
+    if mode == "batched" and sched is None:
+        from .autotune import best_sched
+        sched = best_sched(N, preset or preset_for(N))
     _, _, total = count_tasks(N, rec, mode=mode, sched=sched, order=order, ctx=ctx)
     f90, f180 = ctx.symmetric_counts(N)
     return total, unique_count(total, f90, f180)
