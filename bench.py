@@ -55,8 +55,9 @@ def main() -> int:
     ap.add_argument("--lds", type=int, default=None)
     ap.add_argument("--chunk", type=int, default=None)
     ap.add_argument("--graph", type=int, default=1)
-    ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="fifo")
+    ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="mlp")
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--big-n", type=int, default=21, help="also time one exact count of this N (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration (0 = skip)")
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
@@ -153,6 +154,11 @@ def main() -> int:
                 "kernel": "k_batched", "kernel_ms": round(kavg, 3),
                 "ops_per_node": OPS_PER_NODE, "nodes_per_launch": nodes_total}
 
+    big = None
+    if args.big_n:
+        big = big_count(nqueens, ctx, known, args.big_n, preset, args.scheduler)
+        log(f"[bench] N={args.big_n}: {big}")
+
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(N, rec, nodes, args.cpu_seconds)
@@ -173,7 +179,7 @@ def main() -> int:
         "plain": {"nodes_per_s": round(plain_value, 1), "ms_per_count": round(1000 * plain_wall, 3),
                   "kernel_ms": round(float(np.mean(pk)), 3)},
         "speedup_vs_plain": round(value / plain_value, 3),
-        "roofline": roofline, "cpu_baseline": cpu, "cost_model": mlp_info,
+        "roofline": roofline, "cpu_baseline": cpu, "cost_model": mlp_info, "big_n": big,
         "device": info,
     }
     line = json.dumps(out)
@@ -182,6 +188,32 @@ def main() -> int:
         Path(args.json_out).write_text(line + "\n")
     ctx.close()
     return 0
+
+
+def big_count(nqueens, ctx, known, N: int, preset: int, scheduler: str) -> dict:
+    """One exact count of a larger N (wall time of the count call and its
+    node rate; nodes from a node-counting run of the same set)."""
+    from nqueens import autotune
+    rec = nqueens.gen_constellations(N, preset)
+    order = None
+    if scheduler == "mlp":
+        from nqueens import costmodel
+        order = costmodel.default_model().order(N, rec, ctx=ctx)
+    ctx.load(N, rec, order)
+    s = autotune.best_sched(N, preset)
+    s.use_graph = 1
+    tot, _ = ctx.run_batched(s, count_nodes=True)
+    _, nodes = ctx.results(weighted=False, nodes=True)
+    t0 = time.perf_counter()
+    tot2, t = ctx.run_batched(s)
+    wall = time.perf_counter() - t0
+    want = known["readme_total_unique"][str(N)][0]
+    if tot != want or tot2 != want:
+        raise SystemExit(f"N={N}: wrong total {tot}/{tot2} != {want}")
+    f90, f180 = ctx.symmetric_counts(N)
+    return {"N": N, "preset": preset, "total": tot2, "unique": nqueens.unique_count(tot2, f90, f180),
+            "constellations": int(rec.shape[0]), "nodes": int(nodes.sum()), "wall_s": round(wall, 4),
+            "nodes_per_s": round(float(nodes.sum()) / wall, 1), "split_depth": s.split_depth, "block": s.block}
 
 
 def cpu_baseline(N: int, rec: np.ndarray, nodes: np.ndarray, seconds: float) -> dict:
