@@ -79,6 +79,11 @@ int nq_device_info(nq_ctx *ctx, int32_t *out4);
  * is the processing order for the batched kernel (a permutation of 0..m-1);
  * results are always reported in record order. */
 int nq_ctx_load(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const uint32_t *order);
+/* The same for a set mixing board sizes (N[i] for record i): several count
+ * requests share one resident set and one persistent queue.  No reference
+ * counterpart (the reference counts one N per exec_solutions call,
+ * 18Py:2116-2160). */
+int nq_ctx_load_mixed(nq_ctx *ctx, const int32_t *N, const uint32_t *rec4, uint64_t m, const uint32_t *order);
 
 /* Plain kernel: one lane per constellation, iterative bitboard DFS with the
  * stack in LDS, u64 wave reduction of the weighted counts.  Replaces

@@ -64,12 +64,13 @@ struct nq_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int cu = 0, lds_max = 0, wave = 64, clock_khz = 0;
     // resident task set
-    int N = 0;
+    int N = 0;                    // board size of the set, 0 for a mixed set
     uint64_t m = 0;
     uint32_t Tmax = 0, Tmin = 0;
     bool loaded = false;
     nqk_buf::Buf<uint4> rec, root, sched, items;
     nqk_buf::Buf<uint32_t> order, nitems, off, item_c, err, head;
+    nqk_buf::Buf<uint8_t> nsz;    // board size per record (mixed sets)
     nqk_buf::Buf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
     int sized_split = -1;
     uint64_t item_total = 0;

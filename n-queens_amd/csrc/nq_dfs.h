@@ -19,6 +19,8 @@ struct Sched {
     uint32_t t0, t1, t2, t3, t4;  // parameter bytes for depths 0..15 (t4 unused by the walk)
     uint32_t T;                   // terminal depth
     uint32_t tm;                  // terminal free mask
+    uint32_t mask;                // board mask (1 << N) - 1
+    uint32_t nsh;                 // N - 4: rd block constants are ARc << nsh
 };
 
 // Byte d of the parameter table, for 0 <= d < 16; zero for d >= 16 (the
@@ -48,6 +50,9 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
     S.t0 = a.x; S.t1 = a.y; S.t2 = a.z; S.t3 = a.w; S.t4 = b.x;
     S.T = b.y & 0xFFu;
     S.tm = b.z;
+    const uint32_t N = (b.y >> 16) & 31u;
+    S.mask = (1u << N) - 1u;
+    S.nsh = N - 4u;
     return S;
 }
 

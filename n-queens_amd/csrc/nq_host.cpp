@@ -73,7 +73,7 @@ int elapsed(nq_ctx *c, int a, int b, double *ms)
 int size_items(nq_ctx *c, int split, int block)
 {
     const uint32_t sd = stack_entries(c);
-    NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->N, c->root.p, c->sched.p,
+    NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->root.p, c->sched.p,
                               c->order.p, (uint32_t)c->m, (uint32_t)split, c->nitems.p, nullptr, nullptr,
                               nullptr, nullptr, nullptr));
     NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p));
@@ -99,14 +99,14 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
     NQ_HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
     NQ_HIPCHK(nqk::launch_fill_u32(st, c->head.p, 1, 0));
     if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, m, 0));
-    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->N, c->root.p, c->sched.p, c->order.p, m,
+    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, m,
                               (uint32_t)s.split_depth, c->nitems.p, nullptr, nullptr, nullptr, nullptr, nullptr));
     NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
-    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->N, c->root.p, c->sched.p, c->order.p, m,
+    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, m,
                               (uint32_t)s.split_depth, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
                               c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
-    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sdb, c->N, c->items.p,
+    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sdb, c->items.p,
                                c->item_c.p, c->sched.p, c->item_count.p, c->head.p, (uint32_t)s.chunk,
                                c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[2], st));
@@ -171,10 +171,22 @@ int nq_device_info(nq_ctx *c, int32_t *out4)
     return 0;
 }
 
-int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32_t *order)
+namespace {
+
+// Shared body of nq_ctx_load (one N) and nq_ctx_load_mixed (N per record).
+int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_t m, const uint32_t *order)
 {
     if (int r = check_ctx(c)) return r;
-    if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside the u32 path (6..21)", N);
+    if (!Ns && (N < nqt::kMinN || N > nqt::kMaxN)) return nq_fail(NQ_EINVAL, "N=%d outside the u32 path (6..21)", N);
+    std::vector<uint8_t> n8;
+    if (Ns) {
+        n8.resize(m);
+        for (uint64_t a = 0; a < m; a++) {
+            if (Ns[a] < nqt::kMinN || Ns[a] > nqt::kMaxN)
+                return nq_fail(NQ_EINVAL, "record %llu: N=%d outside the u32 path (6..21)", (unsigned long long)a, Ns[a]);
+            n8[a] = (uint8_t)Ns[a];
+        }
+    }
     if (m && !rec4) return nq_fail(NQ_EINVAL, "null record pointer");
     if (m >= (1ull << 31)) return nq_fail(NQ_EINVAL, "too many records (%llu)", (unsigned long long)m);
     if (order) {
@@ -197,7 +209,8 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
     NQ_HIPCHK(c->weighted.reserve(mm));
     NQ_HIPCHK(c->nodes.reserve(mm));
     NQ_HIPCHK(c->raw.reserve(mm));
-    c->N = N;
+    if (Ns) NQ_HIPCHK(c->nsz.reserve(mm));
+    c->N = Ns ? 0 : N;
     c->m = m;
     std::vector<uint32_t> ord;
     if (!order) {
@@ -208,9 +221,11 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
     if (m) {
         NQ_HIPCHK(hipMemcpyAsync(c->rec.p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
         NQ_HIPCHK(hipMemcpyAsync(c->order.p, order, m * 4, hipMemcpyHostToDevice, c->stream));
+        if (Ns) NQ_HIPCHK(hipMemcpyAsync(c->nsz.p, n8.data(), m, hipMemcpyHostToDevice, c->stream));
     }
     NQ_HIPCHK(nqk::launch_fill_u32(c->stream, c->err.p, 1, 0));
-    NQ_HIPCHK(nqk::launch_decode(c->stream, N, c->rec.p, (uint32_t)m, c->root.p, c->sched.p, c->err.p));
+    NQ_HIPCHK(nqk::launch_decode(c->stream, N, Ns ? c->nsz.p : nullptr, c->rec.p, (uint32_t)m, c->root.p,
+                                 c->sched.p, c->err.p));
     NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->weighted.p, mm, 0));
     NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->nodes.p, mm, 0));
     uint32_t err = 0;
@@ -236,6 +251,19 @@ int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32
     return 0;
 }
 
+}  // namespace
+
+int nq_ctx_load(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const uint32_t *order)
+{
+    return load_impl(c, N, nullptr, rec4, m, order);
+}
+
+int nq_ctx_load_mixed(nq_ctx *c, const int32_t *Ns, const uint32_t *rec4, uint64_t m, const uint32_t *order)
+{
+    if (m && !Ns) return nq_fail(NQ_EINVAL, "null N array");
+    return load_impl(c, 0, Ns, rec4, m, order);
+}
+
 int nq_ctx_run_plain(nq_ctx *c, int count_nodes, uint64_t *total_out, nq_timing *t)
 {
     if (int r = check_ctx(c)) return r;
@@ -248,7 +276,7 @@ int nq_ctx_run_plain(nq_ctx *c, int count_nodes, uint64_t *total_out, nq_timing 
     if (!count_nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, c->m ? c->m : 1, 0));
     NQ_HIPCHK(hipEventRecord(c->ev[0], st));
     NQ_HIPCHK(hipEventRecord(c->ev[1], st));
-    NQ_HIPCHK(nqk::launch_plain(st, count_nodes != 0, block, sd, c->N, c->root.p, c->sched.p, (uint32_t)c->m,
+    NQ_HIPCHK(nqk::launch_plain(st, count_nodes != 0, block, sd, c->root.p, c->sched.p, (uint32_t)c->m,
                              c->weighted.p, c->nodes.p, c->total.p));
     NQ_HIPCHK(hipEventRecord(c->ev[2], st));
     NQ_HIPCHK(hipEventRecord(c->ev[3], st));

@@ -7,19 +7,19 @@ namespace nqk {
 
 constexpr int kStackCap = 20;  // private-memory DFS stack entries (>= T-1 for N <= 21)
 
-hipError_t launch_decode(hipStream_t s, int N, const uint4 *rec, uint32_t m, uint4 *root, uint4 *sched,
-                         uint32_t *err);
-hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, int N, const uint4 *root,
+hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,
+                         uint4 *sched, uint32_t *err);
+hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, const uint4 *root,
                         const uint4 *sched, uint32_t m, unsigned long long *weighted,
                         unsigned long long *nodes_out, unsigned long long *total);
-hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth, int N,
+hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
                          const uint4 *root, const uint4 *sched, const uint32_t *order, uint32_t m, uint32_t split,
                          uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
                          unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off,
                        unsigned long long *count);
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, uint32_t stack_depth,
-                          int N, const uint4 *items, const uint32_t *item_c, const uint4 *sched,
+                          const uint4 *items, const uint32_t *item_c, const uint4 *sched,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
                           unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t batched_occupancy(bool nodes, bool lds_stack, int block, size_t lds_bytes, int *blocks_per_cu);
@@ -30,13 +30,9 @@ hipError_t launch_fill_u32(hipStream_t s, unsigned int *p, uint64_t n, unsigned 
 hipError_t launch_symmetric(hipStream_t s, int N, unsigned long long *out2);
 
 // cost model (nq_mlp.hip)
-hipError_t launch_features(hipStream_t s, int N, const uint4 *rec, const uint4 *root, const uint4 *sched,
+hipError_t launch_features(hipStream_t s, const uint4 *rec, const uint4 *root, const uint4 *sched,
                            uint32_t m, float *feat);
 hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x, uint32_t m, float *y,
                               float *h1, float *h2);
-hipError_t launch_mlp_loss_grad(hipStream_t s, const float *params, const float *x, const float *y,
-                                const uint32_t *rows, uint32_t m, float *grad, float *loss);
-hipError_t launch_adam(hipStream_t s, float *params, const float *grad, float *m1, float *m2, uint32_t n,
-                       float lr, float b1, float b2, float eps, float bc1, float bc2, float scale);
 
 }  // namespace nqk

@@ -40,36 +40,37 @@ __device__ __forceinline__ void run_root(uint32_t mask, uint32_t nsh, const Sche
 
 // ---------------------------------------------------------------------------
 // decode: records -> root frames + schedule descriptors
-// sched layout per record: uint4 {t0,t1,t2,t3}, uint4 {t4, T | wshift<<8, tm, status}
+// sched layout per record: uint4 {t0,t1,t2,t3}, uint4 {t4, T | wshift<<8 | N<<16, tm, status}
 // ---------------------------------------------------------------------------
-__global__ void k_decode(int N, const uint4 *__restrict__ rec, uint32_t m, uint4 *__restrict__ root,
-                         uint4 *__restrict__ sched, uint32_t *__restrict__ err)
+__global__ void k_decode(int N, const uint8_t *__restrict__ Ns, const uint4 *__restrict__ rec, uint32_t m,
+                         uint4 *__restrict__ root, uint4 *__restrict__ sched, uint32_t *__restrict__ err)
 {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m) return;
     const uint4 r = rec[c];
+    const int n = Ns ? (int)Ns[c] : N;
     nqt::Desc D;
-    nqt::decode(N, r.x, r.y, r.z, r.w, D);
+    nqt::decode(n, r.x, r.y, r.z, r.w, D);
     if (D.status != nqt::kOk) atomicOr(err, 1u << D.status);
     root[c] = make_uint4(D.ld, D.rd, D.col, D.avail);
     sched[2 * c] = make_uint4(D.tab[0], D.tab[1], D.tab[2], D.tab[3]);
-    sched[2 * c + 1] = make_uint4(D.tab[4], D.T | (D.wshift << 8), D.tm, (uint32_t)D.status);
+    sched[2 * c + 1] = make_uint4(D.tab[4], D.T | (D.wshift << 8) | ((uint32_t)n << 16), D.tm, (uint32_t)D.status);
 }
 
 // ---------------------------------------------------------------------------
 // plain kernel: one lane per constellation, LDS stack, u64 wave reduction
 // ---------------------------------------------------------------------------
 template <bool NODES>
-__global__ void k_plain(int N, const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
+__global__ void k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
                         unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ nodes_out,
                         unsigned long long *__restrict__ total, uint32_t stack_depth)
 {
     extern __shared__ uint4 lds[];
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
     uint64_t wsum = 0;
     if (c < m) {
         const Sched S = load_sched(sched, c);
+        const uint32_t mask = S.mask, nsh = S.nsh;
         const uint32_t wsh = (sched[2 * c + 1].y >> 8) & 3u;
         LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
         uint64_t cnt, nn;
@@ -89,7 +90,7 @@ __global__ void k_plain(int N, const uint4 *__restrict__ root, const uint4 *__re
 //   at off[q].. ; nodes above the split depth are counted here.
 // ---------------------------------------------------------------------------
 template <bool NODES, bool WRITE>
-__global__ void k_expand(int N, const uint4 *__restrict__ root, const uint4 *__restrict__ sched,
+__global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict__ sched,
                          const uint32_t *__restrict__ order, uint32_t m, uint32_t split,
                          uint32_t *__restrict__ nitems, const uint32_t *__restrict__ off,
                          uint4 *__restrict__ items, uint32_t *__restrict__ item_c,
@@ -99,8 +100,8 @@ __global__ void k_expand(int N, const uint4 *__restrict__ root, const uint4 *__r
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m) return;
     const uint32_t c = order ? order[q] : q;
-    const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
     const Sched S = load_sched(sched, c);
+    const uint32_t mask = S.mask, nsh = S.nsh;
     const uint4 r = root[c];
     LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     uint64_t cnt = 0, nn = 0;
@@ -156,13 +157,12 @@ __global__ void k_scan(const uint32_t *__restrict__ nitems, uint32_t m, uint32_t
 // the wave claims `chunk` items at a time from the global queue head.
 // ---------------------------------------------------------------------------
 template <bool NODES, bool LDS_STACK>
-__global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t *__restrict__ item_c,
+__global__ void k_batched(const uint4 *__restrict__ items, const uint32_t *__restrict__ item_c,
                           const uint4 *__restrict__ sched, const unsigned long long *__restrict__ nitems_total,
                           unsigned int *__restrict__ head, uint32_t chunk,
                           unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
 {
     extern __shared__ uint4 lds[];
-    const uint32_t mask = (1u << N) - 1u, nsh = (uint32_t)(N - 4);
     const uint32_t n_items = (uint32_t)*nitems_total;
     const uint32_t lane = lane_id();
     LdsStack lst{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
@@ -196,7 +196,7 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
                     cnt = 0; nn = 0;
                     if (av != 0u) {
                         if (S.T == 0u) {
-                            cnt = (av & S.tm) ? 1u : 0u;
+                            cnt = (av & S.tm) ? 1u : 0u;   // a root on the terminal row
                         } else {
                             w = Walk{it.x, it.y, it.z, frame_word(av, d0, pbyte(S, d0)), 0u};
                             have = true;
@@ -212,8 +212,8 @@ __global__ void k_batched(int N, const uint4 *__restrict__ items, const uint32_t
         }
         if (exhausted && __ballot(have) == 0ull) break;
         if (!have) continue;
-        const bool done = LDS_STACK ? walk_step<NODES>(mask, nsh, S, w, lst, cnt, nn)
-                                    : walk_step<NODES>(mask, nsh, S, w, pst, cnt, nn);
+        const bool done = LDS_STACK ? walk_step<NODES>(S.mask, S.nsh, S, w, lst, cnt, nn)
+                                    : walk_step<NODES>(S.mask, S.nsh, S, w, pst, cnt, nn);
         if (done) {
             if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
             if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
@@ -361,26 +361,27 @@ __global__ void k_symmetric(int N, int P, uint32_t nprefix, unsigned long long *
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
-hipError_t launch_decode(hipStream_t s, int N, const uint4 *rec, uint32_t m, uint4 *root, uint4 *sched, uint32_t *err)
+hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,
+                         uint4 *sched, uint32_t *err)
 {
     if (m == 0) return hipSuccess;
-    k_decode<<<(m + 255) / 256, 256, 0, s>>>(N, rec, m, root, sched, err);
+    k_decode<<<(m + 255) / 256, 256, 0, s>>>(N, Ns, rec, m, root, sched, err);
     return hipGetLastError();
 }
 
-hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, int N, const uint4 *root,
+hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, const uint4 *root,
                         const uint4 *sched, uint32_t m, unsigned long long *weighted, unsigned long long *nodes_out,
                         unsigned long long *total)
 {
     if (m == 0) return hipSuccess;
     const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
     const dim3 grid((m + block - 1) / block);
-    if (nodes) k_plain<true><<<grid, block, lds, s>>>(N, root, sched, m, weighted, nodes_out, total, stack_depth);
-    else k_plain<false><<<grid, block, lds, s>>>(N, root, sched, m, weighted, nodes_out, total, stack_depth);
+    if (nodes) k_plain<true><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, stack_depth);
+    else k_plain<false><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, stack_depth);
     return hipGetLastError();
 }
 
-hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth, int N,
+hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
                          const uint4 *root, const uint4 *sched, const uint32_t *order, uint32_t m, uint32_t split,
                          uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
                          unsigned long long *raw, unsigned long long *nodes_out)
@@ -389,7 +390,7 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
     const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
     const dim3 grid((m + block - 1) / block);
 #define NQ_EXPAND(NB, WB) \
-    k_expand<NB, WB><<<grid, block, lds, s>>>(N, root, sched, order, m, split, nitems, off, items, item_c, raw, nodes_out)
+    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, m, split, nitems, off, items, item_c, raw, nodes_out)
     if (write) { if (nodes) NQ_EXPAND(true, true); else NQ_EXPAND(false, true); }
     else { if (nodes) NQ_EXPAND(true, false); else NQ_EXPAND(false, false); }
 #undef NQ_EXPAND
@@ -403,13 +404,13 @@ hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32
 }
 
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, uint32_t stack_depth,
-                          int N, const uint4 *items, const uint32_t *item_c, const uint4 *sched,
+                          const uint4 *items, const uint32_t *item_c, const uint4 *sched,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
                           unsigned long long *raw, unsigned long long *nodes_out)
 {
     const size_t lds = lds_stack ? (size_t)block * stack_depth * sizeof(uint4) : 0;
 #define NQ_BATCHED(NB, LB) \
-    k_batched<NB, LB><<<grid, block, lds, s>>>(N, items, item_c, sched, n_items, head, chunk, raw, nodes_out)
+    k_batched<NB, LB><<<grid, block, lds, s>>>(items, item_c, sched, n_items, head, chunk, raw, nodes_out)
     if (lds_stack) { if (nodes) NQ_BATCHED(true, true); else NQ_BATCHED(false, true); }
     else { if (nodes) NQ_BATCHED(true, false); else NQ_BATCHED(false, false); }
 #undef NQ_BATCHED

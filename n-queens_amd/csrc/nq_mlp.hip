@@ -22,12 +22,13 @@ constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH
 // ---------------------------------------------------------------------------
 // features: 24 floats per record, from the record and its decoded descriptor
 // ---------------------------------------------------------------------------
-__global__ void k_features(int N, const uint4 *__restrict__ rec, const uint4 *__restrict__ root,
+__global__ void k_features(const uint4 *__restrict__ rec, const uint4 *__restrict__ root,
                            const uint4 *__restrict__ sched, uint32_t m, float *__restrict__ feat)
 {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m) return;
     const uint4 r = rec[c];
+    const int N = (int)((sched[2 * c + 1].y >> 16) & 31u);
     nqt::Desc D;
     nqt::decode(N, r.x, r.y, r.z, r.w, D);
     const uint4 fr = root[c];
@@ -203,12 +204,11 @@ __global__ void k_adam(float *__restrict__ params, const float *__restrict__ gra
     params[p] -= lr * (a / bc1) / (sqrtf(v / bc2) + eps);
 }
 
-hipError_t launch_features(hipStream_t s, int N, const uint4 *rec, const uint4 *root, const uint4 *sched,
+hipError_t launch_features(hipStream_t s, const uint4 *rec, const uint4 *root, const uint4 *sched,
                            uint32_t m, float *feat)
 {
     if (m == 0) return hipSuccess;
-    (void)sched;
-    k_features<<<(m + 255) / 256, 256, 0, s>>>(N, rec, root, sched, m, feat);
+    k_features<<<(m + 255) / 256, 256, 0, s>>>(rec, root, sched, m, feat);
     return hipGetLastError();
 }
 
@@ -247,7 +247,7 @@ int nq_ctx_features(nq_ctx *c, float *feat_out)
     if (m == 0) return 0;
     nqk_buf::Buf<float> f;
     NQ_HIPCHK(f.reserve(m * nqk::kIn));
-    NQ_HIPCHK(nqk::launch_features(c->stream, c->N, c->rec.p, c->root.p, c->sched.p,
+    NQ_HIPCHK(nqk::launch_features(c->stream, c->rec.p, c->root.p, c->sched.p,
                                    (uint32_t)m, f.p));
     NQ_HIPCHK(hipMemcpyAsync(feat_out, f.p, m * nqk::kIn * sizeof(float), hipMemcpyDeviceToHost,
                              c->stream));

@@ -60,6 +60,7 @@ SIGNATURES = {
     "nq_ctx_destroy": (ctypes.c_int, [_vp]),
     "nq_device_info": (ctypes.c_int, [_vp, _i32p]),
     "nq_ctx_load": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, _vp]),
+    "nq_ctx_load_mixed": (ctypes.c_int, [_vp, _i32p, _u32p, ctypes.c_uint64, _vp]),
     "nq_ctx_run_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Timing)]),
     "nq_ctx_run_batched": (ctypes.c_int, [_vp, ctypes.POINTER(Sched), ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(Timing)]),
@@ -163,6 +164,21 @@ class Context:
             ordp = _ptr(order)
         check(self.lib.nq_ctx_load(self.h, int(N), rec, rec.shape[0], ordp), "nq_ctx_load")
         self.N, self.m = int(N), rec.shape[0]
+
+    def load_mixed(self, Ns, rec, order=None) -> None:
+        """A set mixing board sizes: Ns[i] is the N of record i."""
+        rec = as_records(rec)
+        Ns = np.ascontiguousarray(np.asarray(Ns, dtype=np.int32))
+        if Ns.shape != (rec.shape[0],):
+            raise ValueError("Ns must have one entry per record")
+        ordp = None
+        if order is not None:
+            order = np.ascontiguousarray(np.asarray(order, dtype=np.uint32))
+            if order.shape != (rec.shape[0],):
+                raise ValueError("order must have one entry per record")
+            ordp = _ptr(order)
+        check(self.lib.nq_ctx_load_mixed(self.h, Ns, rec, rec.shape[0], ordp), "nq_ctx_load_mixed")
+        self.N, self.m = 0, rec.shape[0]
 
     def run_plain(self, count_nodes: bool = False) -> tuple[int, Timing]:
         tot = ctypes.c_uint64(0)

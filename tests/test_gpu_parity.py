@@ -137,3 +137,24 @@ def test_empty_set_and_errors(ctx):
     assert e.value.code == -2
     with pytest.raises(nqueens.NQError):
         ctx.load(12, C.gen_constellations(12, 5), order=np.zeros(4300, np.uint32))
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_mixed_board_sizes_in_one_set(ctx, oracle, mode):
+    """Records of several N in one resident set (one persistent queue)."""
+    parts, Ns, want_w, want_n = [], [], [], []
+    for N, preset in [(9, 5), (12, 4), (13, 5), (15, 5), (16, 6), (11, 5)]:
+        rec = C.gen_constellations(N, preset)
+        w, _, nodes = oracle.count_records(N, rec)
+        parts.append(rec); Ns.append(np.full(rec.shape[0], N, np.int32)); want_w.append(w); want_n.append(nodes)
+    rec, Ns = np.concatenate(parts), np.concatenate(Ns)
+    perm = np.random.default_rng(3).permutation(rec.shape[0]).astype(np.uint32)
+    ctx.load_mixed(Ns, rec, order=perm)
+    if mode == "plain":
+        total, _ = ctx.run_plain(count_nodes=True)
+    else:
+        total, _ = ctx.run_batched(sched(3, 128, 1), count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, np.concatenate(want_w))
+    assert np.array_equal(gn, np.concatenate(want_n))
+    assert total == int(np.concatenate(want_w).sum())
