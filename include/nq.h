@@ -47,7 +47,8 @@ typedef struct nq_ctx nq_ctx;
  * one thread per constellation, 18Py:1195-1215, or a grid-stride batch,
  * 14Bit_codon_GPU/359Py_pushbranch_partial.py:1781-1867). */
 typedef struct {
-    int32_t split_depth;   /* frames at this DFS depth become work items (0 = no split) */
+    int32_t split_depth;   /* frames at this DFS depth become work items (0 = no split,
+                              -1 = the per-record depths of nq_ctx_set_splits) */
     int32_t block;         /* workgroup size: 64, 128 or 256 */
     int32_t lds_stack;     /* 1: DFS stack in LDS, 0: in private (scratch) memory */
     int32_t use_graph;     /* 1: replay the launch sequence as a captured hipGraph */
@@ -84,6 +85,13 @@ int nq_ctx_load(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const uint
  * counterpart (the reference counts one N per exec_solutions call,
  * 18Py:2116-2160). */
 int nq_ctx_load_mixed(nq_ctx *ctx, const int32_t *N, const uint32_t *rec4, uint64_t m, const uint32_t *order);
+
+/* Per-record split depths for the batched kernel (used when
+ * sched->split_depth == -1): the cost model bins constellations by predicted
+ * work and splits the heavy ones deeper.  No reference counterpart (the
+ * reference splits every task at one preset depth, 359Py:5776-5785).
+ * Reset by the next load. */
+int nq_ctx_set_splits(nq_ctx *ctx, const uint8_t *depth, uint64_t m);
 
 /* Plain kernel: one lane per constellation, iterative bitboard DFS with the
  * stack in LDS, u64 wave reduction of the weighted counts.  Replaces

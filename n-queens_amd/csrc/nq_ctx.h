@@ -71,8 +71,11 @@ struct nq_ctx {
     nqk_buf::Buf<uint4> rec, root, sched, items;
     nqk_buf::Buf<uint32_t> order, nitems, off, item_c, err, head;
     nqk_buf::Buf<uint8_t> nsz;    // board size per record (mixed sets)
+    nqk_buf::Buf<uint8_t> splits; // split depth per record (nq_ctx_set_splits)
+    bool has_splits = false;
+    uint32_t split_min = 0;       // smallest per-record split depth
     nqk_buf::Buf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
-    int sized_split = -1;
+    int sized_split = -2;         // split depth the item buffer is sized for (-1 per-record, -2 none)
     uint64_t item_total = 0;
     // captured batched sequence
     hipGraph_t graph = nullptr;

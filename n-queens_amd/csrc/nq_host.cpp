@@ -57,6 +57,7 @@ void drop_graph(nq_ctx *c)
 // depth `split` unless a record's T is too small to split (then at its root).
 uint32_t stack_entries(const nq_ctx *c, int split = 0)
 {
+    if (split < 0) split = (int)c->split_min;   // per-record depths: the shallowest item decides
     uint32_t d0 = (split > 0 && (uint32_t)split + 2u <= c->Tmin) ? (uint32_t)split : 0u;
     return c->Tmax > d0 ? c->Tmax - d0 : 1u;
 }
@@ -74,7 +75,8 @@ int size_items(nq_ctx *c, int split, int block)
 {
     const uint32_t sd = stack_entries(c);
     NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->root.p, c->sched.p,
-                              c->order.p, (uint32_t)c->m, (uint32_t)split, c->nitems.p, nullptr, nullptr,
+                              c->order.p, split < 0 ? c->splits.p : nullptr, (uint32_t)c->m,
+                              (uint32_t)(split < 0 ? 0 : split), c->nitems.p, nullptr, nullptr,
                               nullptr, nullptr, nullptr));
     NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p));
     unsigned long long cnt = 0;
@@ -99,11 +101,13 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
     NQ_HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
     NQ_HIPCHK(nqk::launch_fill_u32(st, c->head.p, 1, 0));
     if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, m, 0));
-    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, m,
-                              (uint32_t)s.split_depth, c->nitems.p, nullptr, nullptr, nullptr, nullptr, nullptr));
+    const uint8_t *sp8 = s.split_depth < 0 ? c->splits.p : nullptr;
+    const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
+    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
+                              su, c->nitems.p, nullptr, nullptr, nullptr, nullptr, nullptr));
     NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
-    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, m,
-                              (uint32_t)s.split_depth, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
+    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
+                              su, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
                               c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
     NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sdb, c->items.p,
@@ -198,7 +202,8 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
     }
     drop_graph(c);
     c->loaded = false;
-    c->sized_split = -1;
+    c->sized_split = -2;
+    c->has_splits = false;
     const size_t mm = m ? m : 1;
     NQ_HIPCHK(c->rec.reserve(mm));
     NQ_HIPCHK(c->root.reserve(mm));
@@ -264,6 +269,29 @@ int nq_ctx_load_mixed(nq_ctx *c, const int32_t *Ns, const uint32_t *rec4, uint64
     return load_impl(c, 0, Ns, rec4, m, order);
 }
 
+int nq_ctx_set_splits(nq_ctx *c, const uint8_t *depth, uint64_t m)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
+    if (m != c->m) return nq_fail(NQ_EINVAL, "expected %llu depths, got %llu", (unsigned long long)c->m,
+                                  (unsigned long long)m);
+    if (m && !depth) return nq_fail(NQ_EINVAL, "null depth array");
+    uint32_t mn = 255;
+    for (uint64_t a = 0; a < m; a++) {
+        if (depth[a] > 15) return nq_fail(NQ_EINVAL, "split depth %u > 15 at record %llu", depth[a],
+                                          (unsigned long long)a);
+        mn = std::min<uint32_t>(mn, depth[a]);
+    }
+    NQ_HIPCHK(c->splits.reserve(m ? m : 1));
+    if (m) NQ_HIPCHK(hipMemcpyAsync(c->splits.p, depth, m, hipMemcpyHostToDevice, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    c->has_splits = true;
+    c->split_min = m ? mn : 0;
+    c->sized_split = -2;
+    return 0;
+}
+
 int nq_ctx_run_plain(nq_ctx *c, int count_nodes, uint64_t *total_out, nq_timing *t)
 {
     if (int r = check_ctx(c)) return r;
@@ -301,8 +329,9 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     nq_sched s = *sp;
     if (s.block != 64 && s.block != 128 && s.block != 256) return nq_fail(NQ_EINVAL, "block must be 64, 128 or 256");
     if (s.chunk < 1) s.chunk = 64;
-    if (s.split_depth < 0) return nq_fail(NQ_EINVAL, "negative split depth");
-    // frames at depth T-1 are resolved in place, so items live at depth <= T-2
+    if (s.split_depth < -1) return nq_fail(NQ_EINVAL, "split depth must be >= 0, or -1 for per-record depths");
+    if (s.split_depth == -1 && !c->has_splits) return nq_fail(NQ_ESTATE, "no per-record split depths set");
+    // items live at depth <= T-2 (a deeper split falls back to the root item)
     const int max_split = c->Tmax >= 2 ? (int)c->Tmax - 2 : 0;
     if (s.split_depth > max_split) s.split_depth = max_split;
     const size_t lds_bytes = (size_t)s.block * stack_entries(c, s.split_depth) * 16;

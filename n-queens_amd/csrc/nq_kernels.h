@@ -13,7 +13,8 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_dep
                         const uint4 *sched, uint32_t m, unsigned long long *weighted,
                         unsigned long long *nodes_out, unsigned long long *total);
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
-                         const uint4 *root, const uint4 *sched, const uint32_t *order, uint32_t m, uint32_t split,
+                         const uint4 *root, const uint4 *sched, const uint32_t *order, const uint8_t *splits,
+                         uint32_t m, uint32_t split,
                          uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
                          unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off,

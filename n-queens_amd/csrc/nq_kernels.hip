@@ -85,13 +85,15 @@ __global__ void k_plain(const uint4 *__restrict__ root, const uint4 *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// expansion: frames at depth `split` become work items
+// expansion: frames at depth `split` (one depth for all records, or a depth
+// per record) become work items
 //   pass 0 counts items per position q (processing order), pass 1 writes them
 //   at off[q].. ; nodes above the split depth are counted here.
 // ---------------------------------------------------------------------------
 template <bool NODES, bool WRITE>
 __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict__ sched,
-                         const uint32_t *__restrict__ order, uint32_t m, uint32_t split,
+                         const uint32_t *__restrict__ order, const uint8_t *__restrict__ splits, uint32_t m,
+                         uint32_t split_all,
                          uint32_t *__restrict__ nitems, const uint32_t *__restrict__ off,
                          uint4 *__restrict__ items, uint32_t *__restrict__ item_c,
                          unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
@@ -102,6 +104,7 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     const uint32_t c = order ? order[q] : q;
     const Sched S = load_sched(sched, c);
     const uint32_t mask = S.mask, nsh = S.nsh;
+    const uint32_t split = splits ? (uint32_t)splits[c] : split_all;
     const uint4 r = root[c];
     LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     uint64_t cnt = 0, nn = 0;
@@ -382,7 +385,8 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_dep
 }
 
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
-                         const uint4 *root, const uint4 *sched, const uint32_t *order, uint32_t m, uint32_t split,
+                         const uint4 *root, const uint4 *sched, const uint32_t *order, const uint8_t *splits,
+                         uint32_t m, uint32_t split,
                          uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
                          unsigned long long *raw, unsigned long long *nodes_out)
 {
@@ -390,7 +394,8 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
     const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
     const dim3 grid((m + block - 1) / block);
 #define NQ_EXPAND(NB, WB) \
-    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, m, split, nitems, off, items, item_c, raw, nodes_out)
+    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, splits, m, split, nitems, off, items, item_c, raw, \
+                                              nodes_out)
     if (write) { if (nodes) NQ_EXPAND(true, true); else NQ_EXPAND(false, true); }
     else { if (nodes) NQ_EXPAND(true, false); else NQ_EXPAND(false, false); }
 #undef NQ_EXPAND

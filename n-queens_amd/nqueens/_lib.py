@@ -61,6 +61,8 @@ SIGNATURES = {
     "nq_device_info": (ctypes.c_int, [_vp, _i32p]),
     "nq_ctx_load": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, _vp]),
     "nq_ctx_load_mixed": (ctypes.c_int, [_vp, _i32p, _u32p, ctypes.c_uint64, _vp]),
+    "nq_ctx_set_splits": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"),
+                                         ctypes.c_uint64]),
     "nq_ctx_run_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Timing)]),
     "nq_ctx_run_batched": (ctypes.c_int, [_vp, ctypes.POINTER(Sched), ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(Timing)]),
@@ -179,6 +181,11 @@ class Context:
             ordp = _ptr(order)
         check(self.lib.nq_ctx_load_mixed(self.h, Ns, rec, rec.shape[0], ordp), "nq_ctx_load_mixed")
         self.N, self.m = 0, rec.shape[0]
+
+    def set_splits(self, depth) -> None:
+        """Per-record split depths, used by run_batched with split_depth = -1."""
+        depth = np.ascontiguousarray(np.asarray(depth, dtype=np.uint8))
+        check(self.lib.nq_ctx_set_splits(self.h, depth, depth.size), "nq_ctx_set_splits")
 
     def run_plain(self, count_nodes: bool = False) -> tuple[int, Timing]:
         tot = ctypes.c_uint64(0)

@@ -27,7 +27,8 @@ import numpy as np
 
 from ._lib import NQ_MLP_H, NQ_MLP_IN, NQ_MLP_NPARAM, Context
 
-__all__ = ["CostModel", "init_params", "collect", "train", "default_model", "spearman", "MODEL_PATH"]
+__all__ = ["CostModel", "init_params", "collect", "train", "default_model", "spearman", "binned_splits",
+           "simulated_makespan", "MODEL_PATH"]
 
 MODEL_PATH = Path(__file__).resolve().parent / "costmodel.npz"
 
@@ -123,6 +124,29 @@ def train(ctx: Context, X: np.ndarray, y: np.ndarray, epochs: int = 40, batch: i
         report["holdout_mae"] = float(np.mean(np.abs(pred - y[te])))
         report["holdout_spearman"] = spearman(pred, y[te])
     return model, report
+
+
+def binned_splits(pred_log_nodes: np.ndarray, T: int, target_nodes: float = 4096.0, branching: float = 3.5,
+                  base: int = 1) -> np.ndarray:
+    """Per-record split depths from predicted work: a constellation predicted
+    at n nodes is split at the depth where its items hold about
+    `target_nodes` nodes each (each level divides a subtree by roughly
+    `branching`), never shallower than `base` nor deeper than T-2."""
+    pred = np.expm1(np.asarray(pred_log_nodes, dtype=np.float64))
+    d = np.ceil(np.log(np.maximum(pred / target_nodes, 1.0)) / np.log(branching))
+    return np.clip(d, base, max(T - 2, 0)).astype(np.uint8)
+
+
+def simulated_makespan(costs: np.ndarray, order: np.ndarray, lanes: int) -> float:
+    """Makespan of list scheduling: items taken in `order` by the first free of
+    `lanes` identical workers (the persistent queue's policy), in cost units."""
+    import heapq
+    heap = [0.0] * min(lanes, len(order))
+    heapq.heapify(heap)
+    for i in order:
+        t = heapq.heappop(heap)
+        heapq.heappush(heap, t + float(costs[i]))
+    return max(heap) if heap else 0.0
 
 
 _default = None

@@ -1,0 +1,39 @@
+"""Scheduling value of the cost model, measured without timing noise: the
+makespan of the persistent queue's list scheduling, with measured node counts
+as item costs, for FIFO order vs predicted-cost order (CPU only)."""
+import numpy as np
+
+from nqueens import costmodel
+
+
+def test_simulated_makespan_basics():
+    costs = np.array([5.0, 1, 1, 1, 1, 1])
+    assert costmodel.simulated_makespan(costs, np.arange(6), 2) == 5.0
+    assert costmodel.simulated_makespan(costs, np.array([1, 2, 3, 4, 5, 0]), 2) == 7.0
+
+
+def test_binned_splits_monotone():
+    pred = np.log1p(np.array([10.0, 1e3, 1e4, 1e5, 1e7]))
+    d = costmodel.binned_splits(pred, T=13, target_nodes=1000, branching=4.0, base=1)
+    assert d.dtype == np.uint8 and (np.diff(d.astype(int)) >= 0).all()
+    assert d[0] == 1 and d[-1] <= 11
+
+
+def test_predicted_order_beats_fifo_on_measured_work(oracle):
+    """N=15 constellations with their measured node counts, about five items per
+    lane: heaviest-first by a stand-in predictor (the truth with log-normal
+    noise, Spearman ~0.94; the trained model reaches 0.99) lands within 7 % of
+    the work bound, FIFO 39 % above it."""
+    from nqueens import gen_constellations
+    rec = gen_constellations(15, 5)
+    _, _, nodes = oracle.count_records(15, rec)
+    cost = nodes.astype(np.float64)
+    noisy = np.log1p(cost) + np.random.default_rng(0).normal(0, 0.15, cost.size)
+    assert costmodel.spearman(noisy, cost) > 0.9
+    lanes = 4096
+    fifo = costmodel.simulated_makespan(cost, np.arange(cost.size), lanes)
+    pred = costmodel.simulated_makespan(cost, np.argsort(-noisy, kind="stable"), lanes)
+    best = costmodel.simulated_makespan(cost, np.argsort(-cost, kind="stable"), lanes)
+    lower = cost.sum() / lanes
+    assert best <= pred < fifo
+    assert pred < 1.10 * lower < fifo

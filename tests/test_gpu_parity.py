@@ -158,3 +158,21 @@ def test_mixed_board_sizes_in_one_set(ctx, oracle, mode):
     assert np.array_equal(gw, np.concatenate(want_w))
     assert np.array_equal(gn, np.concatenate(want_n))
     assert total == int(np.concatenate(want_w).sum())
+
+
+def test_per_record_split_depths(ctx, oracle):
+    N = 14
+    rec = C.gen_constellations(N, 5)
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    depth = np.random.default_rng(5).integers(0, 8, rec.shape[0]).astype(np.uint8)   # some beyond T-2
+    ctx.set_splits(depth)
+    for graph in (0, 1, 1):
+        total, t = ctx.run_batched(sched(-1, 128, 1, graph=graph), count_nodes=True)
+        gw, gn = ctx.results(weighted=True, nodes=True)
+        assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+    with pytest.raises(nqueens.NQError):
+        ctx.set_splits(np.zeros(3, np.uint8))
+    ctx.load(N, rec)
+    with pytest.raises(nqueens.NQError):
+        ctx.run_batched(sched(-1, 128, 1))
