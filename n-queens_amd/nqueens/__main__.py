@@ -11,10 +11,7 @@ import sys
 import time
 
 from . import solver
-
-# A000170, as the reference's expected[] (18Py:2089)
-EXPECTED = {6: 4, 7: 40, 8: 92, 9: 352, 10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184,
-            16: 14772512, 17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884, 21: 314666222712}
+from .expected import TOTAL as EXPECTED
 
 
 def fmt(sec: float) -> str:
