@@ -162,16 +162,73 @@ def default_model() -> CostModel:
     return _default
 
 
+def study(ctx: Context, N: int, preset: int = 5, reps: int = 2, block: int = 256, log=print) -> dict:
+    """What the model buys at one N: prediction error against measured node
+    counts, and the batched run time for FIFO, model and true-work orders at
+    uniform split depths, and for model-binned per-record depths."""
+    import statistics
+    from .constellations import gen_constellations
+    from .expected import TOTAL
+    from ._lib import Sched
+    rec = gen_constellations(N, preset)
+    ctx.load(N, rec)
+    ctx.run_batched(Sched(split_depth=3, block=block, lds_stack=1, use_graph=0, chunk=64, waves_per_eu=0),
+                    count_nodes=True)
+    _, nodes = ctx.results(weighted=False, nodes=True)
+    truth = np.log1p(nodes.astype(np.float64))
+    model = default_model()
+    pred = model.predict_features(ctx, ctx.features()).astype(np.float64)
+    out = {"N": N, "preset": preset, "constellations": int(rec.shape[0]), "nodes": int(nodes.sum()),
+           "pred_mae_log": float(np.mean(np.abs(pred - truth))), "pred_spearman": spearman(pred, truth),
+           "runs": []}
+    log(json.dumps({k: v for k, v in out.items() if k != "runs"}))
+    T = N - 1 - preset
+    orders = {"fifo": None, "mlp": np.argsort(-pred, kind="stable").astype(np.uint32),
+              "true": np.argsort(-truth, kind="stable").astype(np.uint32)}
+
+    def timed(label, order, split, depths=None):
+        ctx.load(N, rec, order)
+        if depths is not None:
+            ctx.set_splits(depths)
+        s = Sched(split_depth=split, block=block, lds_stack=1, use_graph=0, chunk=64, waves_per_eu=0)
+        ms, items = [], 0
+        for _ in range(reps + 1):
+            tot, t = ctx.run_batched(s)
+            if tot != TOTAL[N]:
+                raise RuntimeError(f"{label}: wrong total {tot}")
+            ms.append(t.total_ms)
+            items = int(t.items)
+        row = {"config": label, "split": split, "items": items, "run_ms": statistics.median(ms[1:])}
+        out["runs"].append(row)
+        log(json.dumps(row))
+
+    for split in (1, 2, 3):
+        for name, order in orders.items():
+            timed(f"uniform split {split}, {name} order", order, split)
+    for target in (2048.0, 8192.0, 32768.0):
+        for base in (1, 2):
+            d = binned_splits(pred, T, target_nodes=target, base=base)
+            timed(f"mlp bins target {int(target)} base {base}, mlp order", orders["mlp"], -1, d)
+    return out
+
+
 def main(argv=None) -> None:
     """Collect labels at N=16..19 (preset 5), train, report, save
-    (python -m nqueens.costmodel [--out PATH] to also write a copy)."""
+    (python -m nqueens.costmodel [--out PATH] to also write a copy);
+    --study N: compare schedules at N with the saved model."""
     import argparse
     import time
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--study", type=int, default=0)
     a = ap.parse_args(argv)
     ctx = Context(0)
+    if a.study:
+        res = study(ctx, a.study, log=lambda m: print(m, flush=True))
+        if a.out:
+            Path(a.out).write_text(json.dumps(res, indent=1) + "\n")
+        return
     t0 = time.perf_counter()
     X, y, ns = collect(ctx)
     t1 = time.perf_counter()
