@@ -52,7 +52,7 @@ def test_idle_gaps_advance_the_clock():
 @pytest.mark.gpu
 def test_request_loop_on_gpu():
     loop = serve.RequestLoop(max_records=50_000)
-    reqs = serve.poisson_requests(12, 40.0, 10, 15, seed=3)
+    reqs = serve.poisson_requests(12, 5000.0, 10, 15, seed=3)   # arrivals far faster than service
     res = loop.run(reqs)
     assert len(res) == 12 and all(r.ok for r in res)
     assert len({r.batch for r in res}) < 12      # requests shared launches
