@@ -126,15 +126,15 @@ def train(ctx: Context, X: np.ndarray, y: np.ndarray, epochs: int = 40, batch: i
     return model, report
 
 
-def binned_splits(pred_log_nodes: np.ndarray, T: int, target_nodes: float = 4096.0, branching: float = 3.5,
-                  base: int = 1) -> np.ndarray:
-    """Per-record split depths from predicted work: a constellation predicted
-    at n nodes is split at the depth where its items hold about
-    `target_nodes` nodes each (each level divides a subtree by roughly
-    `branching`), never shallower than `base` nor deeper than T-2."""
-    pred = np.expm1(np.asarray(pred_log_nodes, dtype=np.float64))
-    d = np.ceil(np.log(np.maximum(pred / target_nodes, 1.0)) / np.log(branching))
-    return np.clip(d, base, max(T - 2, 0)).astype(np.uint8)
+def binned_splits(pred_log_nodes: np.ndarray, T: int, quantiles=(0.5, 0.8, 0.95), base: int = 1) -> np.ndarray:
+    """Per-record split depths from predicted work: constellations are binned
+    by predicted node count at the given quantiles and each heavier bin is
+    split one level deeper than the last (lightest bin at `base`), never
+    deeper than T-2."""
+    pred = np.asarray(pred_log_nodes, dtype=np.float64)
+    cuts = np.quantile(pred, quantiles) if pred.size else np.zeros(len(quantiles))
+    d = base + np.searchsorted(cuts, pred, side="right")
+    return np.clip(d, 0, max(T - 2, 0)).astype(np.uint8)
 
 
 def simulated_makespan(costs: np.ndarray, order: np.ndarray, lanes: int) -> float:
@@ -205,10 +205,9 @@ def study(ctx: Context, N: int, preset: int = 5, reps: int = 2, block: int = 256
     for split in (1, 2, 3):
         for name, order in orders.items():
             timed(f"uniform split {split}, {name} order", order, split)
-    for target in (2048.0, 8192.0, 32768.0):
-        for base in (1, 2):
-            d = binned_splits(pred, T, target_nodes=target, base=base)
-            timed(f"mlp bins target {int(target)} base {base}, mlp order", orders["mlp"], -1, d)
+    for q, base in (((0.5, 0.8, 0.95), 1), ((0.5, 0.9), 1), ((0.7,), 2), ((0.5, 0.9), 2)):
+        d = binned_splits(pred, T, quantiles=q, base=base)
+        timed(f"mlp bins at quantiles {list(q)} from depth {base}, mlp order", orders["mlp"], -1, d)
     return out
 
 

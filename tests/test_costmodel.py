@@ -13,10 +13,13 @@ def test_simulated_makespan_basics():
 
 
 def test_binned_splits_monotone():
-    pred = np.log1p(np.array([10.0, 1e3, 1e4, 1e5, 1e7]))
-    d = costmodel.binned_splits(pred, T=13, target_nodes=1000, branching=4.0, base=1)
-    assert d.dtype == np.uint8 and (np.diff(d.astype(int)) >= 0).all()
-    assert d[0] == 1 and d[-1] <= 11
+    pred = np.log1p(np.array([10.0, 1e3, 1e4, 1e5, 1e7, 3e7, 5e2, 8e6]))
+    d = costmodel.binned_splits(pred, T=13, quantiles=(0.5, 0.8), base=1)
+    assert d.dtype == np.uint8
+    order = np.argsort(pred)
+    assert (np.diff(d[order].astype(int)) >= 0).all()
+    assert d.min() == 1 and d.max() == 3
+    assert costmodel.binned_splits(pred, T=3, quantiles=(0.5,), base=1).max() == 1
 
 
 def test_predicted_order_beats_fifo_on_measured_work(oracle):
