@@ -13,41 +13,40 @@
 namespace nqk {
 
 // ---------------------------------------------------------------------------
-// per-lane schedule registers
+// per-record schedule (layout of the parameter bytes: nq_task.h)
 // ---------------------------------------------------------------------------
 struct Sched {
-    uint32_t t0, t1, t2, t3, t4;  // parameter bytes for depths 0..15 (t4 unused by the walk)
-    uint32_t T;                   // terminal depth
-    uint32_t tm;                  // terminal free mask
-    uint32_t mask;                // board mask (1 << N) - 1
-    uint32_t nsh;                 // N - 4: rd block constants are ARc << nsh
+    uint32_t t0, t1, t2, t3;  // parameter byte of depth d in byte d of t0..t3 (d < 16)
+    uint32_t T;               // terminal depth (<= 16)
+    uint32_t tm;              // terminal free mask
+    uint32_t mask;            // board mask (1 << N) - 1
+    uint32_t nsh;             // N - 4: rd block constants are ARc << nsh
 };
 
-// Byte d of the parameter table, for 0 <= d < 16; zero for d >= 16 (the
-// terminal depth T <= 16 never needs its own byte).  v_perm_b32 picks byte
-// (d & 7) of a register pair directly, which keeps the table in VGPRs (an
-// indexed select over the words is otherwise lowered to a scratch array).
-// Selector byte 0x0c yields zero, so only byte 0 of each result is non-zero;
-// the pair choice uses masks so that no divergent branch is generated.
-NQ_HD inline uint32_t pbyte(const Sched &S, uint32_t d)
+NQ_HD inline uint32_t tab_word(const Sched &S, uint32_t i)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const uint32_t sel = 0x0c0c0c00u | (d & 7u);
-    const uint32_t a = __builtin_amdgcn_perm(S.t1, S.t0, sel);
-    const uint32_t b = __builtin_amdgcn_perm(S.t3, S.t2, sel);
-    const uint32_t lo = 0u - (uint32_t)(d < 8u);
-    return ((a & lo) | (b & ~lo)) & (0u - (uint32_t)(d < 16u));
-#else
-    const uint32_t w[4] = {S.t0, S.t1, S.t2, S.t3};
-    return d < 16u ? (w[d >> 2] >> ((d & 3u) * 8u)) & 0xFFu : 0u;
-#endif
+    return i == 0u ? S.t0 : i == 1u ? S.t1 : i == 2u ? S.t2 : i == 3u ? S.t3 : 0u;
+}
+
+// Parameter byte of depth d (zero from depth 16 on).
+NQ_HD inline uint32_t qbyte(const Sched &S, uint32_t d)
+{
+    return (tab_word(S, d >> 2) >> ((d & 3u) * 8u)) & 0xFFu;
+}
+
+// Parameter bytes of depths d..d+3 as one little-endian word.
+NQ_HD inline uint32_t qbytes4(const Sched &S, uint32_t d)
+{
+    const uint32_t lo = tab_word(S, d >> 2), hi = tab_word(S, (d >> 2) + 1u);
+    const uint32_t sh = (d & 3u) * 8u;
+    return sh ? (lo >> sh) | (hi << (32u - sh)) : lo;
 }
 
 NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 {
-    uint4 a = sched[2 * c], b = sched[2 * c + 1];
+    const uint4 a = sched[2 * c], b = sched[2 * c + 1];
     Sched S;
-    S.t0 = a.x; S.t1 = a.y; S.t2 = a.z; S.t3 = a.w; S.t4 = b.x;
+    S.t0 = a.x; S.t1 = a.y; S.t2 = a.z; S.t3 = a.w;
     S.T = b.y & 0xFFu;
     S.tm = b.z;
     const uint32_t N = (b.y >> 16) & 31u;
@@ -56,182 +55,207 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
     return S;
 }
 
-// Solutions below a pre-terminal frame (depth T-1) without expanding it.
-// For a placement `bit` the terminal free set is F & ~(bit | bit<<s | bit>>s),
-// F = free set of the terminal row before that placement.  A bit b is a
-// dead end exactly when every f in F is one of b, b<<s, b>>s, i.e. when b is
-// covered by all |F| of the masks F<<s, F, F>>s at once.  The three masks
-// cover a position at most once per f, so the dead bits are: any cover for
-// |F| = 1, the majority of the three masks for |F| = 2, all three for
-// |F| = 3, none for |F| >= 4, and every bit when F is empty.
-NQ_HD inline uint32_t term_count(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
-                                 uint32_t p, uint32_t tm, uint32_t nsh)
+// ---------------------------------------------------------------------------
+// frames
+// ---------------------------------------------------------------------------
+// Frame word: avail (bits 0-18) | parameter byte of the frame's depth
+// (19-26) | depth (27-31).  Depth in the top bits makes "child is on the
+// terminal row" one unsigned compare of W + (1 << 27) against T << 27.
+constexpr uint32_t kAvailMask = 0x7FFFFu;
+constexpr uint32_t kQShift = 19u;
+constexpr uint32_t kDShift = 27u;
+
+NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t d, uint32_t q)
 {
-    const uint32_t s = (p & 3u) + 1u;
-    const uint32_t X = (ld << s) | ((p >> 2) & 3u) | (rd >> s) | (((p >> 4) & 3u) << nsh) | col;
-    const uint32_t F = tm & ~X;
-    const uint32_t a = F << s, c = F >> s;
-    const uint32_t pc = (uint32_t)__builtin_popcount(F);
-    const uint32_t any = a | F | c;
-    const uint32_t maj = (a & F) | (a & c) | (F & c);
-    const uint32_t all = a & F & c;
-    uint32_t B = (pc == 1u) ? any : ((pc == 2u) ? maj : ((pc == 3u) ? all : 0u));
-    B = (pc == 0u) ? ~0u : B;
-    return (uint32_t)__builtin_popcount(avail & ~B);
+    return avail | (q << kQShift) | (d << kDShift);
 }
 
+// Per-lane constants of the walk.  An idle lane (mask 0, T27 all ones)
+// steps without effect: no candidate, no count, no stack motion.
+struct LaneK {
+    uint32_t mask, tm;
+    uint32_t T27;   // terminal depth << 27
+    uint32_t nsh;
+};
+NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.tm, S.T << kDShift, S.nsh}; }
+NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, ~0u, 0u}; }
+
 // ---------------------------------------------------------------------------
-// DFS stacks: LDS (slot i of a lane at base + (i << shift) bytes, i.e. one
-// row of 16-byte entries per slot across the workgroup) or private memory
+// LDS access by 32-bit LDS address (on the device the address goes straight
+// into the ds instruction; on the host it indexes a byte array)
+// ---------------------------------------------------------------------------
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint4 lds_u4;
+struct LdsMem {
+    NQ_HD inline uint4 ld4(uint32_t a) const { return *(const lds_u4 *)(size_t)a; }
+    NQ_HD inline void st4(uint32_t a, uint4 v) const { *(lds_u4 *)(size_t)a = v; }
+    NQ_HD inline uint32_t ld1(uint32_t a) const { return *(const lds_u8 *)(size_t)a; }
+    NQ_HD inline void st32(uint32_t a, uint32_t v) const { *(lds_u32 *)(size_t)a = v; }
+};
+// LDS address of a pointer into the workgroup's shared memory
+__device__ inline uint32_t lds_addr(const void *p) { return (uint32_t)(size_t)(const lds_u8 *)p; }
+#else
+struct LdsMem {
+    uint8_t *mem;
+    NQ_HD inline uint4 ld4(uint32_t a) const { return *(const uint4 *)(mem + a); }
+    NQ_HD inline void st4(uint32_t a, uint4 v) const { *(uint4 *)(mem + a) = v; }
+    NQ_HD inline uint32_t ld1(uint32_t a) const { return mem[a]; }
+    NQ_HD inline void st32(uint32_t a, uint32_t v) const { *(uint32_t *)(mem + a) = v; }
+};
+// for the kernels' host pass only (kernels never run on the host)
+NQ_HD inline uint32_t lds_addr(const void *) { return 0u; }
+#endif
+
+// ---------------------------------------------------------------------------
+// stacks of 16-byte frames {ld, rd, col, frame word}, byte-addressed:
+// slot i of a lane at base + i*S.  The walk keeps the address of its top
+// entry, which is base - S for an empty stack.
 // ---------------------------------------------------------------------------
 struct LdsStack {
-    uint4 *base;       // this lane's slot 0
-    uint32_t shift;    // log2(workgroup size): slot stride in uint4 units
-    NQ_HD inline uint32_t stride() const { return 1u << shift; }
-    NQ_HD inline void put(int i, uint4 v) { base[(uint32_t)i << shift] = v; }
-    NQ_HD inline uint4 get(int i) const { return base[(uint32_t)i << shift]; }
-    // by offset (slot << shift), as the walk keeps it
-    NQ_HD inline void put_at(uint32_t o, uint4 v) { base[o] = v; }
-    NQ_HD inline uint4 get_at(uint32_t o) const { return base[o]; }
+    LdsMem m;
+    uint32_t S;       // slot stride: 16 bytes x workgroup size
+    uint32_t base;    // this lane's slot 0
+    NQ_HD inline uint32_t empty() const { return base - S; }
+    NQ_HD inline uint4 read(uint32_t a) const { return m.ld4(a); }
+    NQ_HD inline void write(uint32_t a, uint4 v) { m.st4(a, v); }
 };
 struct PrivStack {
     uint4 a[kStackCap];
-    NQ_HD inline uint32_t stride() const { return 1u; }
-    NQ_HD inline void put(int i, uint4 v) { a[i] = v; }
-    NQ_HD inline uint4 get(int i) const { return a[i]; }
-    NQ_HD inline void put_at(uint32_t o, uint4 v) { a[o] = v; }
-    NQ_HD inline uint4 get_at(uint32_t o) const { return a[o]; }
+    uint32_t S = 16u;
+    uint32_t base = 0u;
+    NQ_HD inline uint32_t empty() const { return base - S; }
+    NQ_HD inline uint4 read(uint32_t o) const { return a[o >> 4]; }
+    NQ_HD inline void write(uint32_t o, uint4 v) { a[o >> 4] = v; }
 };
 
-// Frame word: avail (bits 0-18) | depth (19-23) | parameter byte (24-31).
-NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t d, uint32_t p)
+// A lane's parameter bytes, one byte per depth from the depth its walk
+// starts at; off = table address minus that depth.
+struct QTab {
+    LdsMem m;
+    uint32_t off;
+    NQ_HD inline uint32_t get(uint32_t d) const { return m.ld1(off + d); }
+};
+
+// Write a lane's parameter bytes of depths d0 .. d0+ts-1 (ts a multiple of 4)
+// at LDS address a.
+NQ_HD inline void write_qtab(const LdsMem &m, uint32_t a, const Sched &S, uint32_t d0, uint32_t ts)
 {
-    return avail | (d << 19) | (p << 24);
+    for (uint32_t k = 0; k < ts; k += 4u) m.st32(a + k, qbytes4(S, d0 + k));
 }
 
-// Stack entry: {ld, rd, col, frame word}.
-NQ_HD inline uint4 pack_frame(uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
-                              uint32_t d, uint32_t p)
+// A lane's DFS position: the current frame and the address of the top
+// stack entry.
+struct Walk {
+    uint32_t ld, rd, col, W;
+    uint32_t A;
+};
+
+// One DFS node, without divergent branches.  Every lane extracts one
+// candidate of its frame and builds the child (18Py:368-562 one iteration):
+// shift by the step, OR the block constants, drop column 0 when the child is
+// the jmark frame, apply the one-row lookahead.  A child on the terminal row
+// (depth T) counts one solution when its free set meets the terminal mask.
+// The lane then descends (pushing the frame's remainder when it has one),
+// keeps the remainder, or pops.  The top entry is read every step (used only
+// when popping), so its load is issued a full step before its use.  Frames
+// with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots.
+// Solutions are counted in a u32; `wrap` reports the step whose count
+// wrapped to zero (the caller adds 2^32).  Returns true when the walk's
+// subtree is exhausted.
+template <bool NODES, class Stack>
+NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, uint32_t &cnt, uint64_t &nodes,
+                            bool &wrap)
 {
-    return make_uint4(ld, rd, col, frame_word(avail, d, p));
+    const uint32_t S = st.S;
+    const uint4 top = st.read((uint32_t)max((int32_t)w.A, (int32_t)st.base));
+    const uint32_t W = w.W;
+    const uint32_t bit = W & (0u - W);                   // lowest candidate (avail is the low field)
+    const uint32_t rest = W ^ bit;
+    const uint32_t s = (W >> 19) & 3u;
+    const uint32_t nld = ((w.ld | bit) << s) | ((W >> 21) & 3u);
+    const uint32_t nrd = ((w.rd | bit) >> s) | (((W >> 23) & 3u) << L.nsh);
+    const uint32_t ncol = w.col | bit;
+    const uint32_t occ = nld | nrd | ((W >> 26) & 1u);   // JMN: column 0 of a jmark child
+    const uint32_t nf = L.mask & ~(occ | ncol);
+    const uint32_t nft = nf & L.tm;
+    const uint32_t nla = (uint32_t)((int32_t)(W << 6) >> 31);   // all ones when the frame has no prune
+    const bool lok = ((L.mask & ~((nld << 1) | (nrd >> 1) | ncol)) | nla) != 0u;
+    const uint32_t Wd = W + (1u << kDShift);             // child depth in the top bits
+    const uint32_t qc = qt.get(Wd >> kDShift);
+    const bool leaf = Wd >= L.T27;
+    uint32_t nc;
+    wrap = __builtin_add_overflow(cnt, (uint32_t)(leaf & lok & (nft != 0u)), &nc);
+    cnt = nc;
+    if (NODES) nodes += 1u;
+    const bool down = lok & (nf != 0u) & !leaf;
+    const bool more = (rest & kAvailMask) != 0u;
+    const bool push = down & more;
+    const bool pop = !down & !more & (w.A != st.empty());
+    if (push) st.write(w.A + S, make_uint4(w.ld, w.rd, w.col, rest));
+    const uint32_t Wc = (Wd & (31u << kDShift)) | (qc << kQShift) | nf;
+    w.ld = down ? nld : (pop ? top.x : w.ld);
+    w.rd = down ? nrd : (pop ? top.y : w.rd);
+    w.col = down ? ncol : (pop ? top.z : w.col);
+    w.W = down ? Wc : (pop ? top.w : rest);
+    w.A += push ? S : (pop ? 0u - S : 0u);
+    return !down & !more & !pop;
 }
 
-// One DFS from a frame at depth d (d <= T-2, avail != 0 after jmark) until
-// its subtree is exhausted, handing children created at depth `split` to
-// emit() instead of expanding them.  Used by the work-item expansion.
-template <bool NODES, bool SPLIT, class Stack, class Emit>
-NQ_HD inline void dfs_subtree(uint32_t mask, uint32_t nsh, const Sched &S, uint32_t ld,
-                              uint32_t rd, uint32_t col, uint32_t avail, uint32_t d,
-                              uint32_t split, Stack &st, uint64_t &cnt, uint64_t &nodes,
-                              Emit &&emit)
+// ---------------------------------------------------------------------------
+// branchy DFS for the work-item expansion
+// ---------------------------------------------------------------------------
+// Index-addressed stack: slot i of a lane at p[i << shift].
+struct SlotStack {
+    uint4 *p;
+    uint32_t shift;
+    NQ_HD inline void put(int i, uint4 v) { p[(uint32_t)i << shift] = v; }
+    NQ_HD inline uint4 get(int i) const { return p[(uint32_t)i << shift]; }
+};
+
+// One DFS from a frame at depth d (< T) until its subtree is exhausted,
+// handing the children created at depth `split` (d < split < T) to emit()
+// instead of expanding them.  Counts the solutions and nodes above the split.
+template <bool NODES, class Stack, class Emit>
+NQ_HD inline void expand_subtree(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32_t avail,
+                                 uint32_t d, uint32_t split, Stack &st, uint64_t &cnt, uint64_t &nodes,
+                                 Emit &&emit)
 {
-    uint32_t p = pbyte(S, d);
-    const uint32_t Tm1 = S.T - 1u;
+    uint32_t q = qbyte(S, d);
     int sp = 0;
     for (;;) {
-        const uint32_t s = (p & 3u) + 1u;
         const uint32_t bit = avail & (0u - avail);
         avail ^= bit;
-        uint32_t nld = ((ld | bit) << s) | ((p >> 2) & 3u);
-        const uint32_t nrd = ((rd | bit) >> s) | (((p >> 4) & 3u) << nsh);
+        const uint32_t s = q & 3u;
+        const uint32_t nld = ((ld | bit) << s) | ((q >> 2) & 3u);
+        const uint32_t nrd = ((rd | bit) >> s) | (((q >> 4) & 3u) << S.nsh);
         const uint32_t ncol = col | bit;
-        uint32_t nf = mask & ~(nld | nrd | ncol);
-        const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
+        const uint32_t occ = nld | nrd | ncol | ((q >> 7) & 1u);
+        const uint32_t nf = S.mask & ~occ;
+        const bool lok = ((q >> 6) & 1u) != 0u || (S.mask & ~((nld << 1) | (nrd >> 1) | ncol)) != 0u;
         const uint32_t cd = d + 1u;
-        const uint32_t cp = pbyte(S, cd);
-        bool ok = (nf != 0u) && (((p & 0x40u) == 0u) || (look != 0u));
-        if (cp & 0x80u) { nf &= ~1u; nld |= 1u; }
-        ok = ok && (nf != 0u);
         if (NODES) nodes += 1u;
-        if (cd == Tm1) {
-            if (ok) {
-                cnt += term_count(nld, nrd, ncol, nf, cp, S.tm, nsh);
-                if (NODES) nodes += (uint32_t)__builtin_popcount(nf);
-            }
-            ok = false;
+        bool down = false;
+        if (cd == S.T) {
+            if (lok && (S.tm & ~occ) != 0u) cnt += 1u;
+        } else if (lok && nf != 0u) {
+            if (cd == split) emit(nld, nrd, ncol, nf);
+            else down = true;
         }
-        if (SPLIT && ok && cd == split) {
-            emit(nld, nrd, ncol, nf);
-            ok = false;
-        }
-        if (ok) {
-            if (avail) { st.put(sp, pack_frame(ld, rd, col, avail, d, p)); sp++; }
-            ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; p = cp;
+        if (down) {
+            if (avail) { st.put(sp, make_uint4(ld, rd, col, avail | (d << kDShift))); sp++; }
+            ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; q = qbyte(S, cd);
         } else if (avail == 0u) {
             if (sp == 0) break;
             sp--;
             const uint4 e = st.get(sp);
             ld = e.x; rd = e.y; col = e.z;
-            avail = e.w & 0x7FFFFu;
-            d = (e.w >> 19) & 31u;
-            p = e.w >> 24;
+            avail = e.w & kAvailMask;
+            d = e.w >> kDShift;
+            q = qbyte(S, d);
         }
     }
 }
-
-// A lane's DFS position: the current frame {ld, rd, col, frame word} and
-// the offset of the first free stack slot (entries below it x slot stride).
-struct Walk {
-    uint32_t ld, rd, col, W;
-    uint32_t so;
-};
-
-// One DFS node, without divergent branches.  Every lane extracts one
-// candidate of its frame and builds the child; a child on the terminal row
-// (depth T) counts one solution when its free set meets the terminal mask
-// (the base frame of 18Py:226-234: functionid 14 ignores column bit 0).
-// The lane then descends, keeps its frame, or pops.  The frame remainder is
-// written to the free slot `sp` every step (kept only when descending) and
-// the entry `sp-1` is read every step (used only when popping), so the LDS
-// traffic is uniform and the load is issued a full step ahead of its use.
-// Needs one stack slot more than the deepest push.  Returns true when the
-// walk's subtree is exhausted.
-template <bool NODES, class Stack>
-NQ_HD inline bool walk_step(uint32_t mask, uint32_t nsh, const Sched &S, Walk &w, Stack &st, uint64_t &cnt,
-                            uint64_t &nodes)
-{
-    const uint32_t stride = st.stride();
-    const uint4 top = st.get_at((uint32_t)max((int)w.so - (int)stride, 0));
-    const uint32_t W = w.W;
-    const uint32_t avail = W & 0x7FFFFu;
-    const uint32_t bit = avail & (0u - avail);
-    const uint32_t rest = W ^ bit;                       // remainder, same depth and parameters
-    const uint32_t s = ((W >> 24) & 3u) + 1u;
-    uint32_t nld = ((w.ld | bit) << s) | ((W >> 26) & 3u);
-    const uint32_t nrd = ((w.rd | bit) >> s) | (((W >> 28) & 3u) << nsh);
-    const uint32_t ncol = w.col | bit;
-    uint32_t nf = mask & ~(nld | nrd | ncol);
-    const uint32_t look = mask & ~((nld << 1) | (nrd >> 1) | ncol);
-    const uint32_t cd = ((W >> 19) & 31u) + 1u;
-    const uint32_t cp = pbyte(S, cd);
-    const uint32_t nla = ((W >> 30) & 1u) - 1u;          // all ones when the frame has no lookahead
-    const uint32_t jm = cp >> 7;                         // jmark frame: clear column 0, ld |= 1
-    const bool ok0 = (nf != 0u) & ((look | nla) != 0u);
-    nf &= ~jm;
-    nld |= jm;
-    const bool ok = ok0 & (nf != 0u);
-    const bool leaf = (cd == S.T);
-    cnt += (uint32_t)(leaf & ok & ((nf & S.tm) != 0u));
-    if (NODES) nodes += 1u;
-    st.put_at(w.so, make_uint4(w.ld, w.rd, w.col, rest));
-    const bool down = ok & !leaf;
-    const bool more = (rest & 0x7FFFFu) != 0u;
-    const bool pop = !down & !more & (w.so != 0u);
-    const uint32_t Wc = nf | (cd << 19) | (cp << 24);
-    // next frame = child (down), popped entry (pop) or this frame's remainder,
-    // selected with masks so that all lanes run the same instructions
-    const uint32_t dm = 0u - (uint32_t)down, pm = 0u - (uint32_t)pop;
-    w.ld = (nld & dm) | (((top.x & pm) | (w.ld & ~pm)) & ~dm);
-    w.rd = (nrd & dm) | (((top.y & pm) | (w.rd & ~pm)) & ~dm);
-    w.col = (ncol & dm) | (((top.z & pm) | (w.col & ~pm)) & ~dm);
-    w.W = (Wc & dm) | (((top.w & pm) | (rest & ~pm)) & ~dm);
-    w.so += (down & more) ? stride : (pop ? (0u - stride) : 0u);
-    return !down && !more && !pop;
-}
-
-struct NoEmit {
-    NQ_HD inline void operator()(uint32_t, uint32_t, uint32_t, uint32_t) const {}
-};
-
 
 }  // namespace nqk

@@ -51,15 +51,20 @@ void drop_graph(nq_ctx *c)
     c->gkey = nq_graph_key{};
 }
 
-// Stack slots per lane for a walk from depth d0: frames with remaining
-// candidates sit at depths d0..T-2 (at most T-1-d0 entries) and the walk
-// writes one slot past the top, so T-d0 slots suffice.  Items start at
-// depth `split` unless a record's T is too small to split (then at its root).
-uint32_t stack_entries(const nq_ctx *c, int split = 0)
+// Stack slots per lane of the expansion DFS (frames of depths 0..T-2).
+uint32_t expand_slots(const nq_ctx *c)
+{
+    return c->Tmax > 1u ? c->Tmax : 1u;
+}
+
+// LDS layout of the batched walk: items start at depth `split` unless a
+// record's T is too small to split (then at its root), so the shallowest
+// start depth decides the table and stack sizes.
+nqk::WalkLds batched_lds(const nq_ctx *c, int split)
 {
     if (split < 0) split = (int)c->split_min;   // per-record depths: the shallowest item decides
-    uint32_t d0 = (split > 0 && (uint32_t)split + 2u <= c->Tmin) ? (uint32_t)split : 0u;
-    return c->Tmax > d0 ? c->Tmax - d0 : 1u;
+    const uint32_t d0 = (split > 0 && (uint32_t)split + 2u <= c->Tmin) ? (uint32_t)split : 0u;
+    return nqk::walk_lds(c->Tmax, d0);
 }
 
 int elapsed(nq_ctx *c, int a, int b, double *ms)
@@ -73,7 +78,7 @@ int elapsed(nq_ctx *c, int a, int b, double *ms)
 // Item count of the split at `split` (expansion count pass + scan, read back).
 int size_items(nq_ctx *c, int split, int block)
 {
-    const uint32_t sd = stack_entries(c);
+    const uint32_t sd = expand_slots(c);
     NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->root.p, c->sched.p,
                               c->order.p, split < 0 ? c->splits.p : nullptr, (uint32_t)c->m,
                               (uint32_t)(split < 0 ? 0 : split), c->nitems.p, nullptr, nullptr,
@@ -94,7 +99,8 @@ int size_items(nq_ctx *c, int split, int block)
 // DFS, finalize.  Events 0/3 bracket the sequence, 1/2 the DFS kernel.
 int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool events)
 {
-    const uint32_t m = (uint32_t)c->m, sd = stack_entries(c), sdb = stack_entries(c, s.split_depth);
+    const uint32_t m = (uint32_t)c->m, sd = expand_slots(c);
+    const nqk::WalkLds lay = batched_lds(c, s.split_depth);
     hipStream_t st = c->stream;
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[0], st));
     NQ_HIPCHK(nqk::launch_fill_u64(st, c->raw.p, m, 0));
@@ -110,7 +116,7 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
                               su, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
                               c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
-    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, sdb, c->items.p,
+    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, lay, c->items.p,
                                c->item_c.p, c->sched.p, c->item_count.p, c->head.p, (uint32_t)s.chunk,
                                c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[2], st));
@@ -297,14 +303,14 @@ int nq_ctx_run_plain(nq_ctx *c, int count_nodes, uint64_t *total_out, nq_timing 
     if (int r = check_ctx(c)) return r;
     if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
     const int block = 64;
-    const uint32_t sd = stack_entries(c);
-    if ((size_t)block * sd * 16 > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack too large");
+    const nqk::WalkLds lay = nqk::walk_lds(c->Tmax, 0u);
+    if (lay.bytes(block) > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack too large");
     hipStream_t st = c->stream;
     NQ_HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
     if (!count_nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, c->m ? c->m : 1, 0));
     NQ_HIPCHK(hipEventRecord(c->ev[0], st));
     NQ_HIPCHK(hipEventRecord(c->ev[1], st));
-    NQ_HIPCHK(nqk::launch_plain(st, count_nodes != 0, block, sd, c->root.p, c->sched.p, (uint32_t)c->m,
+    NQ_HIPCHK(nqk::launch_plain(st, count_nodes != 0, block, lay, c->root.p, c->sched.p, (uint32_t)c->m,
                              c->weighted.p, c->nodes.p, c->total.p));
     NQ_HIPCHK(hipEventRecord(c->ev[2], st));
     NQ_HIPCHK(hipEventRecord(c->ev[3], st));
@@ -334,7 +340,8 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     // items live at depth <= T-2 (a deeper split falls back to the root item)
     const int max_split = c->Tmax >= 2 ? (int)c->Tmax - 2 : 0;
     if (s.split_depth > max_split) s.split_depth = max_split;
-    const size_t lds_bytes = (size_t)s.block * stack_entries(c, s.split_depth) * 16;
+    const nqk::WalkLds lay = batched_lds(c, s.split_depth);
+    const size_t lds_bytes = s.lds_stack ? lay.bytes(s.block) : (size_t)s.block * lay.ts;
     if (lds_bytes > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, c->lds_max);
     if (c->m == 0) {
         if (total_out) *total_out = 0;
@@ -346,7 +353,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         if (int r = size_items(c, s.split_depth, s.block)) return r;
     }
     int per_cu = 0;
-    NQ_HIPCHK(nqk::batched_occupancy(count_nodes != 0, s.lds_stack != 0, s.block, s.lds_stack ? lds_bytes : 0, &per_cu));
+    NQ_HIPCHK(nqk::batched_occupancy(count_nodes != 0, s.lds_stack != 0, s.block, lds_bytes, &per_cu));
     if (per_cu < 1) per_cu = 1;
     if (s.waves_per_eu > 0) {
         int want = (s.waves_per_eu * 4 * 64) / s.block;   // 4 SIMDs per CU

@@ -5,11 +5,27 @@
 
 namespace nqk {
 
-constexpr int kStackCap = 20;  // private-memory DFS stack entries (>= T-1 for N <= 21)
+constexpr int kStackCap = 16;  // private-memory DFS stack entries (>= T-1, T <= 16)
+
+// LDS of a walking workgroup: `ts` parameter-table bytes per lane, then the
+// stack, `slots` entries of 16 bytes per lane.  A walk from depth d0 reads
+// the bytes of depths d0..T (the read of the terminal depth is unused but
+// stays in the lane's bytes: ts > T-d0) and pushes at most T-1-d0 frames.
+// An odd word count per lane keeps the byte reads of a wave on distinct banks.
+struct WalkLds {
+    uint32_t ts, slots;
+    size_t bytes(int block) const { return (size_t)block * (ts + 16u * slots); }
+};
+inline WalkLds walk_lds(uint32_t Tmax, uint32_t d0)
+{
+    const uint32_t rows = Tmax > d0 ? Tmax - d0 : 1u;
+    const uint32_t words = (rows / 4u + 1u) | 1u;
+    return WalkLds{4u * words, Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u};
+}
 
 hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,
                          uint4 *sched, uint32_t *err);
-hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, const uint4 *root,
+hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const uint4 *root,
                         const uint4 *sched, uint32_t m, unsigned long long *weighted,
                         unsigned long long *nodes_out, unsigned long long *total);
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
@@ -19,7 +35,7 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
                          unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off,
                        unsigned long long *count);
-hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, uint32_t stack_depth,
+hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,
                           const uint4 *items, const uint32_t *item_c, const uint4 *sched,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
                           unsigned long long *raw, unsigned long long *nodes_out);

@@ -25,18 +25,27 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Count + nodes of a whole constellation from its descriptor.
+// Count + nodes of a whole constellation from its root frame (depth 0).
 template <bool NODES, class Stack>
-__device__ __forceinline__ void run_root(uint32_t mask, uint32_t nsh, const Sched &S, uint4 root, Stack &st,
-                                         uint64_t &cnt, uint64_t &nodes)
+__device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, const QTab &qt, uint64_t &cnt,
+                                         uint64_t &nodes)
 {
     cnt = 0; nodes = 0;
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
-    Walk w{root.x, root.y, root.z, frame_word(root.w, 0u, pbyte(S, 0)), 0u};
-    while (!walk_step<NODES>(mask, nsh, S, w, st, cnt, nodes)) {
-    }
+    const LaneK L = lane_consts(S);
+    Walk w{root.x, root.y, root.z, frame_word(root.w, 0u, qbyte(S, 0u)), st.empty()};
+    uint32_t lo = 0;
+    uint64_t hi = 0;
+    bool wrap, done;
+    do {
+        done = walk_step<NODES>(L, w, st, qt, lo, nodes, wrap);
+        if (wrap) hi += 1ull << 32;
+    } while (!done);
+    cnt = hi + lo;
 }
+
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // ---------------------------------------------------------------------------
 // decode: records -> root frames + schedule descriptors
@@ -63,23 +72,25 @@ __global__ void k_decode(int N, const uint8_t *__restrict__ Ns, const uint4 *__r
 template <bool NODES>
 __global__ void k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
                         unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ nodes_out,
-                        unsigned long long *__restrict__ total, uint32_t stack_depth)
+                        unsigned long long *__restrict__ total, uint32_t ts)
 {
     extern __shared__ uint4 lds[];
+    const uint32_t lb = lds_addr(lds);
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t wsum = 0;
     if (c < m) {
         const Sched S = load_sched(sched, c);
-        const uint32_t mask = S.mask, nsh = S.nsh;
         const uint32_t wsh = (sched[2 * c + 1].y >> 8) & 3u;
-        LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
+        const uint32_t tab = lb + threadIdx.x * ts;
+        write_qtab(LdsMem{}, tab, S, 0u, ts);
+        LdsStack st{LdsMem{}, 16u * blockDim.x, lb + ts * blockDim.x + 16u * threadIdx.x};
+        const QTab qt{LdsMem{}, tab};
         uint64_t cnt, nn;
-        run_root<NODES>(mask, nsh, S, root[c], st, cnt, nn);
+        run_root<NODES>(S, root[c], st, qt, cnt, nn);
         wsum = cnt << wsh;
         weighted[c] = wsum;
         if (NODES) nodes_out[c] = nn;
     }
-    (void)stack_depth;
     wsum = wave_sum_u64(wsum);
     if (lane_id() == 0 && wsum) atomicAdd(total, (unsigned long long)wsum);
 }
@@ -103,10 +114,9 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     if (q >= m) return;
     const uint32_t c = order ? order[q] : q;
     const Sched S = load_sched(sched, c);
-    const uint32_t mask = S.mask, nsh = S.nsh;
     const uint32_t split = splits ? (uint32_t)splits[c] : split_all;
     const uint4 r = root[c];
-    LdsStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
+    SlotStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     uint64_t cnt = 0, nn = 0;
     uint32_t n = 0;
     const uint32_t base = WRITE ? off[q] : 0u;
@@ -123,7 +133,7 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
                 }
                 n++;
             };
-            dfs_subtree<NODES, true>(mask, nsh, S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
+            expand_subtree<NODES>(S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
         }
     }
     if (!WRITE) nitems[q] = n;
@@ -159,69 +169,89 @@ __global__ void k_scan(const uint32_t *__restrict__ nitems, uint32_t m, uint32_t
 // work item from the wave's claimed range as soon as its subtree is done;
 // the wave claims `chunk` items at a time from the global queue head.
 // ---------------------------------------------------------------------------
+template <bool LDS_STACK> struct BatchedStack;
+template <> struct BatchedStack<true> { using type = LdsStack; };
+template <> struct BatchedStack<false> { using type = PrivStack; };
+
+template <bool NODES>
+__device__ __forceinline__ void flush_counts(unsigned long long *raw, unsigned long long *nodes_out, uint32_t c,
+                                             uint32_t cnt, uint64_t nn)
+{
+    if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
+    if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
+}
+
 template <bool NODES, bool LDS_STACK>
 __global__ void k_batched(const uint4 *__restrict__ items, const uint32_t *__restrict__ item_c,
                           const uint4 *__restrict__ sched, const unsigned long long *__restrict__ nitems_total,
-                          unsigned int *__restrict__ head, uint32_t chunk,
+                          unsigned int *__restrict__ head, uint32_t chunk, uint32_t ts,
                           unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
 {
     extern __shared__ uint4 lds[];
+    const uint32_t lb = lds_addr(lds);
     const uint32_t n_items = (uint32_t)*nitems_total;
     const uint32_t lane = lane_id();
-    LdsStack lst{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
-    PrivStack pst;
+    const uint32_t tab = lb + threadIdx.x * ts;
+    typename BatchedStack<LDS_STACK>::type st;
+    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * blockDim.x, lb + ts * blockDim.x + 16u * threadIdx.x};
+    QTab qt{LdsMem{}, tab};
+    LaneK L = idle_lane();
+    Walk w{0u, 0u, 0u, 0u, st.empty()};
     uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
     bool exhausted = false;           // wave-uniform: queue drained
     bool have = false;                // this lane holds a subtree
-    uint32_t c = 0;
-    Walk w{0u, 0u, 0u, 0u, 0u};
-    Sched S{};
-    uint64_t cnt = 0, nn = 0;
+    uint32_t c = 0, cnt = 0;
+    uint64_t nn = 0;
     for (;;) {
-        const uint64_t need = __ballot(!have);
-        if (need != 0ull && !exhausted) {
+        // every lane steps; idle lanes step without effect
+        bool wrap;
+        const bool fin = walk_step<NODES>(L, w, st, qt, cnt, nn, wrap) && have;
+        if (wave_ballot(fin || wrap || !have) == 0ull) continue;
+        if (wrap) atomicAdd(raw + c, 1ull << 32);
+        if (fin) {
+            flush_counts<NODES>(raw, nodes_out, c, cnt, nn);
+            have = false;
+            L = idle_lane();
+            w.W = 0u;
+            qt.off = tab;
+        }
+        // hand the next items of the wave's claimed range to the lanes without one
+        while (!exhausted) {
+            const uint64_t need = wave_ballot(!have);
+            if (need == 0ull) break;
             if (next >= end) {
                 uint32_t b = 0;
                 if (lane == 0) b = atomicAdd(head, chunk);
-                b = __shfl(b, 0, 64);
-                if (b >= n_items) exhausted = true;
-                else { next = b; end = min(b + chunk, n_items); }
+                b = __builtin_amdgcn_readfirstlane(__shfl(b, 0, 64));
+                if (b >= n_items) { exhausted = true; break; }
+                next = b;
+                end = min(b + chunk, n_items);
             }
-            if (!exhausted) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
-                                          __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-                const uint32_t idx = next + rank;
-                if (!have && idx < end) {
-                    const uint4 it = items[idx];
-                    c = item_c[idx];
-                    S = load_sched(sched, c);
-                    const uint32_t d0 = it.w >> 19, av = it.w & 0x7FFFFu;
-                    cnt = 0; nn = 0;
-                    if (av != 0u) {
-                        if (S.T == 0u) {
-                            cnt = (av & S.tm) ? 1u : 0u;   // a root on the terminal row
-                        } else {
-                            w = Walk{it.x, it.y, it.z, frame_word(av, d0, pbyte(S, d0)), 0u};
-                            have = true;
-                        }
-                    }
-                    if (!have) {
-                        if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
-                        if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+            const uint32_t idx = next + rank;
+            if (!have && idx < end) {
+                const uint4 it = items[idx];
+                c = item_c[idx];
+                const Sched S = load_sched(sched, c);
+                const uint32_t d0 = it.w >> 19, av = it.w & kAvailMask;
+                cnt = 0; nn = 0;
+                if (av != 0u) {
+                    if (S.T == 0u) {
+                        // a root on the terminal row
+                        if (av & S.tm) atomicAdd(raw + c, 1ull);
+                    } else {
+                        write_qtab(LdsMem{}, tab, S, d0, ts);
+                        qt.off = tab - d0;
+                        L = lane_consts(S);
+                        w = Walk{it.x, it.y, it.z, frame_word(av, d0, qbyte(S, d0)), st.empty()};
+                        have = true;
                     }
                 }
-                next = min(next + (uint32_t)__builtin_popcountll(need), end);
             }
+            next = min(next + (uint32_t)__builtin_popcountll(need), end);
         }
-        if (exhausted && __ballot(have) == 0ull) break;
-        if (!have) continue;
-        const bool done = LDS_STACK ? walk_step<NODES>(S.mask, S.nsh, S, w, lst, cnt, nn)
-                                    : walk_step<NODES>(S.mask, S.nsh, S, w, pst, cnt, nn);
-        if (done) {
-            if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
-            if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
-            have = false;
-        }
+        if (wave_ballot(have) == 0ull) break;   // reached only once the queue is drained
     }
 }
 
@@ -372,15 +402,15 @@ hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *r
     return hipGetLastError();
 }
 
-hipError_t launch_plain(hipStream_t s, bool nodes, int block, uint32_t stack_depth, const uint4 *root,
+hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const uint4 *root,
                         const uint4 *sched, uint32_t m, unsigned long long *weighted, unsigned long long *nodes_out,
                         unsigned long long *total)
 {
     if (m == 0) return hipSuccess;
-    const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
+    const size_t lds = lay.bytes(block);
     const dim3 grid((m + block - 1) / block);
-    if (nodes) k_plain<true><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, stack_depth);
-    else k_plain<false><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, stack_depth);
+    if (nodes) k_plain<true><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, lay.ts);
+    else k_plain<false><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, lay.ts);
     return hipGetLastError();
 }
 
@@ -408,14 +438,14 @@ hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, uint32_t stack_depth,
+hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,
                           const uint4 *items, const uint32_t *item_c, const uint4 *sched,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
                           unsigned long long *raw, unsigned long long *nodes_out)
 {
-    const size_t lds = lds_stack ? (size_t)block * stack_depth * sizeof(uint4) : 0;
+    const size_t lds = lds_stack ? lay.bytes(block) : (size_t)block * lay.ts;
 #define NQ_BATCHED(NB, LB) \
-    k_batched<NB, LB><<<grid, block, lds, s>>>(items, item_c, sched, n_items, head, chunk, raw, nodes_out)
+    k_batched<NB, LB><<<grid, block, lds, s>>>(items, item_c, sched, n_items, head, chunk, lay.ts, raw, nodes_out)
     if (lds_stack) { if (nodes) NQ_BATCHED(true, true); else NQ_BATCHED(false, true); }
     else { if (nodes) NQ_BATCHED(true, false); else NQ_BATCHED(false, false); }
 #undef NQ_BATCHED

@@ -61,7 +61,7 @@ __global__ void k_features(const uint4 *__restrict__ rec, const uint4 *__restric
     f[20] = (float)D.T * inv;
     f[21] = (float)N * (1.0f / 21.0f);
     int marks = 0;
-    for (int d = 0; d < (int)D.T; d++) marks += ((nqt::param_byte(D.tab, d) & 3u) != 0u);
+    for (int d = 0; d < (int)D.T; d++) marks += ((nqt::param_byte(D.tab, d) & 3u) > 1u);
     f[22] = (float)marks * 0.5f;
     f[23] = lsum * 0.1f;
 }

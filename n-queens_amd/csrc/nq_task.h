@@ -7,14 +7,20 @@
 // 18Py:1103-1156).  In the reference every frame looks its behaviour up from
 // (functionid, row); but functionid changes only at fixed rows, so the
 // behaviour of a frame is a function of its DFS depth alone.  The decode walks
-// the functionid/row sequence once and stores one parameter byte per depth:
+// the functionid/row sequence once and stores one parameter byte per depth d,
+// describing the move from a frame at depth d to its child:
 //
-//   bits 0-1  s-1   rows advanced by a placement (1, or 2/3 on a mark row)
-//   bits 2-3  AL    constant OR-ed into the child's ld (blockL | add1)
+//   bits 0-1  s     rows advanced by a placement (1, or 2/3 on a mark row)
+//   bits 2-3  AL    constant OR-ed into the child's ld after the shift
+//                   (blockL | add1); on the jmark frame AL = 2 instead, which
+//                   is the reference's "ld |= 1" on that frame shifted by its
+//                   step of 1 (a jmark frame is never a mark frame)
 //   bits 4-5  ARc   constant OR-ed into the child's rd: ARc << (N-4), i.e.
 //                   0, 1<<(N-4) or 1<<(N-3)
-//   bit  6    LA    one-row lookahead prune (availptn and row+1 < endmark)
-//   bit  7    JM    this frame is the jmark frame: clear column bit 0, ld |= 1
+//   bit  6    NLA   no one-row lookahead prune at this depth (the prune runs
+//                   when availptn is set and row+1 < endmark)
+//   bit  7    JMN   the child is the jmark frame: column bit 0 is not one of
+//                   its candidates (18Py:419-421)
 //
 // T is the depth of the terminal row (row == endmark on a base functionid);
 // a child created at depth T counts one solution when its free set is
@@ -200,10 +206,12 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     D.wshift = weight_shift(ijkl, N);
     // ---- per-depth schedule: the functionid/row walk of dfs_iter ----
     int f = target, row = start, T = -1;
+    uint8_t jm[kMaxDepth + 1] = {};
+    uint8_t qb[kMaxDepth] = {};
     for (int d = 0; d < kMaxDepth; d++) {
         const int ptn = meta_ptn(f);
         if (ptn == 5 && row == endmark) { T = d; break; }
-        uint32_t s = 1, AL = 0, ARc = 0, LA = 0, JM = 0;
+        uint32_t s = 1, AL = 0, ARc = 0, LA = 0;
         if (ptn == 0 || ptn == 1 || ptn == 2) {
             const bool at = (ptn == 1) ? (row == mark2) : (row == mark1);
             if (at) {
@@ -214,7 +222,8 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
             }
             // mark functionids have availptn 0: no lookahead on their plain rows
         } else if (ptn == 3 && row == jmark) {
-            JM = 1;
+            jm[d] = 1;
+            AL = 2;
             LA = (meta_avail(f) && row + 1 < endmark) ? 1u : 0u;
             f = meta_next(f);
         } else if (ptn == 4) {
@@ -222,16 +231,18 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
         } else {
             LA = (meta_avail(f) && row + 1 < endmark) ? 1u : 0u;
         }
-        D.tab[d >> 2] |= ((s - 1) | (AL << 2) | (ARc << 4) | (LA << 6) | (JM << 7)) << ((d & 3) * 8);
+        qb[d] = (uint8_t)(s | (AL << 2) | (ARc << 4) | ((LA ^ 1u) << 6));
         row += (int)s;
         if (row > N + 3) break;
     }
+    for (int d = 0; d < kMaxDepth; d++)
+        D.tab[d >> 2] |= (uint32_t)(qb[d] | (jm[d + 1] << 7)) << ((d & 3) * 8);
     if (T < 0) { D.status = kErrNoTerminal; return; }
     if (T >= kMaxDepth - 1) { D.status = kErrDepth; return; }
     D.T = (uint32_t)T;
     D.tm = (f == 14) ? (mask & ~1u) : mask;
     // root frame, jmark applied when the root itself is the jmark frame
-    if (param_byte(D.tab, 0) & 0x80u) { fr &= ~1u; ld |= 1u; }
+    if (jm[0]) { fr &= ~1u; ld |= 1u; }
     D.ld = ld; D.rd = rd; D.col = col; D.avail = fr;
 }
 

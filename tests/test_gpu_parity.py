@@ -176,3 +176,34 @@ def test_per_record_split_depths(ctx, oracle):
     ctx.load(N, rec)
     with pytest.raises(nqueens.NQError):
         ctx.run_batched(sched(-1, 128, 1))
+
+
+@pytest.fixture(scope="module")
+def sample_n20_n21():
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "oracle_sample_n20_n21.npz") as z:
+        return {k: z[k] for k in z.files}
+
+
+# The largest boards: every 400th record of N=20/21 (cached oracle counts).
+# Long ld histories fill all 32 bits of the lookahead's occupancy word here,
+# which N <= 19 rarely does; the prune switch must not depend on a spare bit.
+@pytest.mark.parametrize("N", [20, 21])
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_largest_n_oracle_sample(ctx, sample_n20_n21, N, mode):
+    rec = sample_n20_n21[f"rec{N}"]
+    ctx.load(N, rec)
+    if mode == "plain":
+        total, _ = ctx.run_plain(count_nodes=True)
+    else:
+        total, _ = ctx.run_batched(sched(3, 256, 1), count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, sample_n20_n21[f"w{N}"])
+    assert np.array_equal(gn, sample_n20_n21[f"nodes{N}"])
+    assert total == int(sample_n20_n21[f"w{N}"].sum())
+
+
+@pytest.mark.parametrize("N", [20, 21])
+def test_largest_n_full_count(ctx, known, N):
+    total, unique = nqueens.count(N, ctx=ctx)
+    assert [total, unique] == list(known["readme_total_unique"][str(N)])
