@@ -69,7 +69,7 @@ struct nq_ctx {
     uint32_t Tmax = 0, Tmin = 0;
     bool loaded = false;
     nqk_buf::Buf<uint4> rec, root, sched, items;
-    nqk_buf::Buf<uint32_t> order, nitems, off, item_c, err, head;
+    nqk_buf::Buf<uint32_t> order, nitems, off, err, head;
     nqk_buf::Buf<uint8_t> nsz;    // board size per record (mixed sets)
     nqk_buf::Buf<uint8_t> splits; // split depth per record (nq_ctx_set_splits)
     bool has_splits = false;

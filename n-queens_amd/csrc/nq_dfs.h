@@ -13,43 +13,45 @@
 namespace nqk {
 
 // ---------------------------------------------------------------------------
-// per-record schedule (layout of the parameter bytes: nq_task.h)
+// per-record schedule (layout: nqt::sched_words)
 // ---------------------------------------------------------------------------
 struct Sched {
-    uint32_t t0, t1, t2, t3;  // parameter byte of depth d in byte d of t0..t3 (d < 16)
+    uint32_t e[8];            // 16-bit entries: frame-word upper half of depth T-1-r
     uint32_t T;               // terminal depth (<= 16)
     uint32_t tm;              // terminal free mask
     uint32_t mask;            // board mask (1 << N) - 1
     uint32_t nsh;             // N - 4: rd block constants are ARc << nsh
 };
 
-NQ_HD inline uint32_t tab_word(const Sched &S, uint32_t i)
+NQ_HD inline uint32_t sched_word(const Sched &S, uint32_t i)
 {
-    return i == 0u ? S.t0 : i == 1u ? S.t1 : i == 2u ? S.t2 : i == 3u ? S.t3 : 0u;
+    uint32_t v = S.e[0];
+#pragma unroll
+    for (uint32_t k = 1; k < 8u; k++) v = (i == k) ? S.e[k] : v;
+    return v;
 }
 
-// Parameter byte of depth d (zero from depth 16 on).
+// Entry r (r < 16): the upper half of the frame word of depth T-1-r.
+NQ_HD inline uint32_t entry(const Sched &S, uint32_t r)
+{
+    return (sched_word(S, r >> 1) >> ((r & 1u) * 16u)) & 0xFFFFu;
+}
+
+// Parameter byte of depth d < T.
 NQ_HD inline uint32_t qbyte(const Sched &S, uint32_t d)
 {
-    return (tab_word(S, d >> 2) >> ((d & 3u) * 8u)) & 0xFFu;
-}
-
-// Parameter bytes of depths d..d+3 as one little-endian word.
-NQ_HD inline uint32_t qbytes4(const Sched &S, uint32_t d)
-{
-    const uint32_t lo = tab_word(S, d >> 2), hi = tab_word(S, (d >> 2) + 1u);
-    const uint32_t sh = (d & 3u) * 8u;
-    return sh ? (lo >> sh) | (hi << (32u - sh)) : lo;
+    return (entry(S, S.T - 1u - d) >> 3) & 0xFFu;
 }
 
 NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 {
-    const uint4 a = sched[2 * c], b = sched[2 * c + 1];
+    const uint4 a = sched[kSchedU4 * c], b = sched[kSchedU4 * c + 1], i = sched[kSchedU4 * c + 2];
     Sched S;
-    S.t0 = a.x; S.t1 = a.y; S.t2 = a.z; S.t3 = a.w;
-    S.T = b.y & 0xFFu;
-    S.tm = b.z;
-    const uint32_t N = (b.y >> 16) & 31u;
+    S.e[0] = a.x; S.e[1] = a.y; S.e[2] = a.z; S.e[3] = a.w;
+    S.e[4] = b.x; S.e[5] = b.y; S.e[6] = b.z; S.e[7] = b.w;
+    S.T = i.x & 0xFFu;
+    S.tm = i.y;
+    const uint32_t N = (i.x >> 16) & 31u;
     S.mask = (1u << N) - 1u;
     S.nsh = N - 4u;
     return S;
@@ -59,26 +61,27 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // frames
 // ---------------------------------------------------------------------------
 // Frame word: avail (bits 0-18) | parameter byte of the frame's depth
-// (19-26) | depth (27-31).  Depth in the top bits makes "child is on the
-// terminal row" one unsigned compare of W + (1 << 27) against T << 27.
+// (19-26) | 0 (27) | r = T-1-depth (28-31).  The frame's children are on the
+// terminal row exactly when r = 0, i.e. when the word is below 2^28, and
+// (W - 2^28) >> 27 is twice the child's r: the byte offset of the child's
+// entry in the lane's table.
 constexpr uint32_t kAvailMask = 0x7FFFFu;
 constexpr uint32_t kQShift = 19u;
-constexpr uint32_t kDShift = 27u;
+constexpr uint32_t kRShift = 28u;
 
-NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t d, uint32_t q)
+NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t r, uint32_t q)
 {
-    return avail | (q << kQShift) | (d << kDShift);
+    return avail | (q << kQShift) | (r << kRShift);
 }
 
-// Per-lane constants of the walk.  An idle lane (mask 0, T27 all ones)
-// steps without effect: no candidate, no count, no stack motion.
+// Per-lane constants of the walk.  An idle lane (mask 0) steps without
+// effect: no candidate, no count, no stack motion.
 struct LaneK {
-    uint32_t mask, tm;
-    uint32_t T27;   // terminal depth << 27
+    uint32_t mask;
     uint32_t nsh;
 };
-NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.tm, S.T << kDShift, S.nsh}; }
-NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, ~0u, 0u}; }
+NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.nsh}; }
+NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u}; }
 
 // ---------------------------------------------------------------------------
 // LDS access by 32-bit LDS address (on the device the address goes straight
@@ -86,12 +89,13 @@ NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, ~0u, 0u}; }
 // ---------------------------------------------------------------------------
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint4 lds_u4;
 struct LdsMem {
     NQ_HD inline uint4 ld4(uint32_t a) const { return *(const lds_u4 *)(size_t)a; }
     NQ_HD inline void st4(uint32_t a, uint4 v) const { *(lds_u4 *)(size_t)a = v; }
-    NQ_HD inline uint32_t ld1(uint32_t a) const { return *(const lds_u8 *)(size_t)a; }
+    NQ_HD inline uint32_t ld2(uint32_t a) const { return *(const lds_u16 *)(size_t)a; }
     NQ_HD inline void st32(uint32_t a, uint32_t v) const { *(lds_u32 *)(size_t)a = v; }
 };
 // LDS address of a pointer into the workgroup's shared memory
@@ -101,7 +105,7 @@ struct LdsMem {
     uint8_t *mem;
     NQ_HD inline uint4 ld4(uint32_t a) const { return *(const uint4 *)(mem + a); }
     NQ_HD inline void st4(uint32_t a, uint4 v) const { *(uint4 *)(mem + a) = v; }
-    NQ_HD inline uint32_t ld1(uint32_t a) const { return mem[a]; }
+    NQ_HD inline uint32_t ld2(uint32_t a) const { return *(const uint16_t *)(mem + a); }
     NQ_HD inline void st32(uint32_t a, uint32_t v) const { *(uint32_t *)(mem + a) = v; }
 };
 // for the kernels' host pass only (kernels never run on the host)
@@ -130,19 +134,21 @@ struct PrivStack {
     NQ_HD inline void write(uint32_t o, uint4 v) { a[o >> 4] = v; }
 };
 
-// A lane's parameter bytes, one byte per depth from the depth its walk
-// starts at; off = table address minus that depth.
+// A lane's frame-word upper halves indexed by r: entry r (at byte 2r) is
+// the child frame at depth T-1-r.
 struct QTab {
     LdsMem m;
     uint32_t off;
-    NQ_HD inline uint32_t get(uint32_t d) const { return m.ld1(off + d); }
+    NQ_HD inline uint32_t at(uint32_t byte_off) const { return m.ld2(off + byte_off); }
 };
 
-// Write a lane's parameter bytes of depths d0 .. d0+ts-1 (ts a multiple of 4)
-// at LDS address a.
-NQ_HD inline void write_qtab(const LdsMem &m, uint32_t a, const Sched &S, uint32_t d0, uint32_t ts)
+// Write a lane's first ts/4 schedule words (ts a multiple of 4, <= 32) at
+// LDS address a.
+NQ_HD inline void write_qtab(const LdsMem &m, uint32_t a, const Sched &S, uint32_t ts)
 {
-    for (uint32_t k = 0; k < ts; k += 4u) m.st32(a + k, qbytes4(S, d0 + k));
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++)
+        if (4u * k < ts) m.st32(a + 4u * k, S.e[k]);
 }
 
 // A lane's DFS position: the current frame and the address of the top
@@ -156,17 +162,14 @@ struct Walk {
 // candidate of its frame and builds the child (18Py:368-562 one iteration):
 // shift by the step, OR the block constants, drop column 0 when the child is
 // the jmark frame, apply the one-row lookahead.  A child on the terminal row
-// (depth T) counts one solution when its free set meets the terminal mask.
+// (depth T) counts one solution when its free set is not empty.
 // The lane then descends (pushing the frame's remainder when it has one),
 // keeps the remainder, or pops.  The top entry is read every step (used only
 // when popping), so its load is issued a full step before its use.  Frames
 // with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots.
-// Solutions are counted in a u32; `wrap` reports the step whose count
-// wrapped to zero (the caller adds 2^32).  Returns true when the walk's
-// subtree is exhausted.
-template <bool NODES, class Stack>
-NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, uint32_t &cnt, uint64_t &nodes,
-                            bool &wrap)
+// Returns true when the walk's subtree is exhausted.
+template <bool NODES, class Count, class Stack>
+NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, Count &cnt, uint64_t &nodes)
 {
     const uint32_t S = st.S;
     const uint4 top = st.read((uint32_t)max((int32_t)w.A, (int32_t)st.base));
@@ -179,22 +182,24 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, 
     const uint32_t ncol = w.col | bit;
     const uint32_t occ = nld | nrd | ((W >> 26) & 1u);   // JMN: column 0 of a jmark child
     const uint32_t nf = L.mask & ~(occ | ncol);
-    const uint32_t nft = nf & L.tm;
     const uint32_t nla = (uint32_t)((int32_t)(W << 6) >> 31);   // all ones when the frame has no prune
     const bool lok = ((L.mask & ~((nld << 1) | (nrd >> 1) | ncol)) | nla) != 0u;
-    const uint32_t Wd = W + (1u << kDShift);             // child depth in the top bits
-    const uint32_t qc = qt.get(Wd >> kDShift);
-    const bool leaf = Wd >= L.T27;
-    uint32_t nc;
-    wrap = __builtin_add_overflow(cnt, (uint32_t)(leaf & lok & (nft != 0u)), &nc);
-    cnt = nc;
+    const uint32_t Wd = W - (1u << kRShift);             // child's r in the top bits
+    const uint32_t hc = qt.at(Wd >> 27);                 // child frame word, upper half
+    const bool inner = W >= (1u << kRShift);             // children not on the terminal row
+    const bool ok = lok & (nf != 0u);
+    const bool down = ok & inner;
+    const bool sol = ok ^ down;
+    cnt += (Count)sol;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(cnt));   // keeps the add beside its condition (one add-with-carry)
+#endif
     if (NODES) nodes += 1u;
-    const bool down = lok & (nf != 0u) & !leaf;
     const bool more = (rest & kAvailMask) != 0u;
     const bool push = down & more;
     const bool pop = !down & !more & (w.A != st.empty());
     if (push) st.write(w.A + S, make_uint4(w.ld, w.rd, w.col, rest));
-    const uint32_t Wc = (Wd & (31u << kDShift)) | (qc << kQShift) | nf;
+    const uint32_t Wc = (hc << 16) + nf;              // disjoint fields
     w.ld = down ? nld : (pop ? top.x : w.ld);
     w.rd = down ? nrd : (pop ? top.y : w.rd);
     w.col = down ? ncol : (pop ? top.z : w.col);
@@ -244,7 +249,7 @@ NQ_HD inline void expand_subtree(const Sched &S, uint32_t ld, uint32_t rd, uint3
             else down = true;
         }
         if (down) {
-            if (avail) { st.put(sp, make_uint4(ld, rd, col, avail | (d << kDShift))); sp++; }
+            if (avail) { st.put(sp, make_uint4(ld, rd, col, avail | (d << 24))); sp++; }
             ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; q = qbyte(S, cd);
         } else if (avail == 0u) {
             if (sp == 0) break;
@@ -252,7 +257,7 @@ NQ_HD inline void expand_subtree(const Sched &S, uint32_t ld, uint32_t rd, uint3
             const uint4 e = st.get(sp);
             ld = e.x; rd = e.y; col = e.z;
             avail = e.w & kAvailMask;
-            d = e.w >> kDShift;
+            d = e.w >> 24;
             q = qbyte(S, d);
         }
     }

@@ -82,14 +82,13 @@ int size_items(nq_ctx *c, int split, int block)
     NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->root.p, c->sched.p,
                               c->order.p, split < 0 ? c->splits.p : nullptr, (uint32_t)c->m,
                               (uint32_t)(split < 0 ? 0 : split), c->nitems.p, nullptr, nullptr,
-                              nullptr, nullptr, nullptr));
+                              nullptr, nullptr));
     NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p));
     unsigned long long cnt = 0;
     NQ_HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     NQ_HIPCHK(hipStreamSynchronize(c->stream));
     if (cnt >= (1ull << 32)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^32)", split, cnt);
-    NQ_HIPCHK(c->items.reserve(cnt));
-    NQ_HIPCHK(c->item_c.reserve(cnt));
+    NQ_HIPCHK(c->items.reserve(nqk::kItemU4 * cnt));
     c->item_total = cnt;
     c->sized_split = split;
     return 0;
@@ -110,14 +109,14 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
     const uint8_t *sp8 = s.split_depth < 0 ? c->splits.p : nullptr;
     const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
     NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
-                              su, c->nitems.p, nullptr, nullptr, nullptr, nullptr, nullptr));
+                              su, c->nitems.p, nullptr, nullptr, nullptr, nullptr));
     NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
     NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
-                              su, c->nitems.p, c->off.p, c->items.p, c->item_c.p,
+                              su, c->nitems.p, c->off.p, c->items.p,
                               c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
     NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, lay, c->items.p,
-                               c->item_c.p, c->sched.p, c->item_count.p, c->head.p, (uint32_t)s.chunk,
+                               c->item_count.p, c->head.p, (uint32_t)s.chunk,
                                c->raw.p, c->nodes.p));
     if (events) NQ_HIPCHK(hipEventRecord(c->ev[2], st));
     NQ_HIPCHK(nqk::launch_finalize(st, c->sched.p, m, c->raw.p, c->weighted.p, c->total.p));
@@ -213,7 +212,7 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
     const size_t mm = m ? m : 1;
     NQ_HIPCHK(c->rec.reserve(mm));
     NQ_HIPCHK(c->root.reserve(mm));
-    NQ_HIPCHK(c->sched.reserve(2 * mm));
+    NQ_HIPCHK(c->sched.reserve(nqk::kSchedU4 * mm));
     NQ_HIPCHK(c->order.reserve(mm));
     NQ_HIPCHK(c->nitems.reserve(mm));
     NQ_HIPCHK(c->off.reserve(mm));
@@ -247,12 +246,12 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
     // the terminal depth is N-1-preset for every record of a set; take the max
     c->Tmax = 0;
     if (m) {
-        std::vector<uint4> sc(2 * m);
-        NQ_HIPCHK(hipMemcpy(sc.data(), c->sched.p, 2 * m * 16, hipMemcpyDeviceToHost));
+        std::vector<uint4> sc(nqk::kSchedU4 * m);
+        NQ_HIPCHK(hipMemcpy(sc.data(), c->sched.p, nqk::kSchedU4 * m * 16, hipMemcpyDeviceToHost));
         c->Tmin = 0xFFu;
         for (uint64_t a = 0; a < m; a++) {
-            c->Tmax = std::max(c->Tmax, sc[2 * a + 1].y & 0xFFu);
-            c->Tmin = std::min(c->Tmin, sc[2 * a + 1].y & 0xFFu);
+            c->Tmax = std::max(c->Tmax, sc[nqk::kSchedU4 * a + 2].x & 0xFFu);
+            c->Tmin = std::min(c->Tmin, sc[nqk::kSchedU4 * a + 2].x & 0xFFu);
         }
     }
     // the walk keeps 16 parameter bytes per record (depths 0..15), so the

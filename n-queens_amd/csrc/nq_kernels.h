@@ -6,21 +6,24 @@
 namespace nqk {
 
 constexpr int kStackCap = 16;  // private-memory DFS stack entries (>= T-1, T <= 16)
+constexpr uint32_t kSchedU4 = 3;   // uint4 words of a record's schedule (nqt::sched_words)
+// A work item, 4 x uint4: {ld, rd, col, frame word}, {record, T | wshift << 8 |
+// N << 16, terminal mask, 0}, and the record's schedule words 0-7.
+constexpr uint32_t kItemU4 = 4;
 
-// LDS of a walking workgroup: `ts` parameter-table bytes per lane, then the
-// stack, `slots` entries of 16 bytes per lane.  A walk from depth d0 reads
-// the bytes of depths d0..T (the read of the terminal depth is unused but
-// stays in the lane's bytes: ts > T-d0) and pushes at most T-1-d0 frames.
-// An odd word count per lane keeps the byte reads of a wave on distinct banks.
+// LDS of a walking workgroup: `ts` table bytes per lane (16-bit entries of
+// the children's frame-word upper halves by r = T-1-depth; a walk from depth
+// d0 uses r < T-1-d0, and its reads for terminal children land at byte 30,
+// inside the LDS that follows), then the stack, `slots` entries of 16 bytes
+// per lane (at most T-1-d0 frames are pushed).
 struct WalkLds {
     uint32_t ts, slots;
     size_t bytes(int block) const { return (size_t)block * (ts + 16u * slots); }
 };
 inline WalkLds walk_lds(uint32_t Tmax, uint32_t d0)
 {
-    const uint32_t rows = Tmax > d0 ? Tmax - d0 : 1u;
-    const uint32_t words = (rows / 4u + 1u) | 1u;
-    return WalkLds{4u * words, Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u};
+    const uint32_t entries = Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u;
+    return WalkLds{4u * ((entries + 1u) / 2u), entries};
 }
 
 hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,
@@ -31,12 +34,12 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
                          const uint4 *root, const uint4 *sched, const uint32_t *order, const uint8_t *splits,
                          uint32_t m, uint32_t split,
-                         uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
+                         uint32_t *nitems, const uint32_t *off, uint4 *items,
                          unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off,
                        unsigned long long *count);
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,
-                          const uint4 *items, const uint32_t *item_c, const uint4 *sched,
+                          const uint4 *items,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
                           unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t batched_occupancy(bool nodes, bool lds_stack, int block, size_t lds_bytes, int *blocks_per_cu);

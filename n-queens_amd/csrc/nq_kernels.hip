@@ -34,22 +34,16 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
-    Walk w{root.x, root.y, root.z, frame_word(root.w, 0u, qbyte(S, 0u)), st.empty()};
-    uint32_t lo = 0;
-    uint64_t hi = 0;
-    bool wrap, done;
-    do {
-        done = walk_step<NODES>(L, w, st, qt, lo, nodes, wrap);
-        if (wrap) hi += 1ull << 32;
-    } while (!done);
-    cnt = hi + lo;
+    Walk w{root.x, root.y, root.z, root.w | (entry(S, S.T - 1u) << 16), st.empty()};
+    while (!walk_step<NODES>(L, w, st, qt, cnt, nodes)) {
+    }
 }
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // ---------------------------------------------------------------------------
 // decode: records -> root frames + schedule descriptors
-// sched layout per record: uint4 {t0,t1,t2,t3}, uint4 {t4, T | wshift<<8 | N<<16, tm, status}
+// sched layout per record: nqt::sched_words
 // ---------------------------------------------------------------------------
 __global__ void k_decode(int N, const uint8_t *__restrict__ Ns, const uint4 *__restrict__ rec, uint32_t m,
                          uint4 *__restrict__ root, uint4 *__restrict__ sched, uint32_t *__restrict__ err)
@@ -62,28 +56,34 @@ __global__ void k_decode(int N, const uint8_t *__restrict__ Ns, const uint4 *__r
     nqt::decode(n, r.x, r.y, r.z, r.w, D);
     if (D.status != nqt::kOk) atomicOr(err, 1u << D.status);
     root[c] = make_uint4(D.ld, D.rd, D.col, D.avail);
-    sched[2 * c] = make_uint4(D.tab[0], D.tab[1], D.tab[2], D.tab[3]);
-    sched[2 * c + 1] = make_uint4(D.tab[4], D.T | (D.wshift << 8) | ((uint32_t)n << 16), D.tm, (uint32_t)D.status);
+    uint32_t w[12];
+    nqt::sched_words(D, n, w);
+    sched[kSchedU4 * c] = make_uint4(w[0], w[1], w[2], w[3]);
+    sched[kSchedU4 * c + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    sched[kSchedU4 * c + 2] = make_uint4(w[8], w[9], w[10], w[11]);
 }
 
 // ---------------------------------------------------------------------------
 // plain kernel: one lane per constellation, LDS stack, u64 wave reduction
 // ---------------------------------------------------------------------------
+constexpr uint32_t kPlainBlock = 64;
+
 template <bool NODES>
-__global__ void k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
-                        unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ nodes_out,
-                        unsigned long long *__restrict__ total, uint32_t ts)
+__global__ void __launch_bounds__(kPlainBlock)
+k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
+        unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ nodes_out,
+        unsigned long long *__restrict__ total, uint32_t ts)
 {
     extern __shared__ uint4 lds[];
     const uint32_t lb = lds_addr(lds);
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.x * kPlainBlock + threadIdx.x;
     uint64_t wsum = 0;
     if (c < m) {
         const Sched S = load_sched(sched, c);
-        const uint32_t wsh = (sched[2 * c + 1].y >> 8) & 3u;
+        const uint32_t wsh = (sched[kSchedU4 * c + 2].x >> 8) & 3u;
         const uint32_t tab = lb + threadIdx.x * ts;
-        write_qtab(LdsMem{}, tab, S, 0u, ts);
-        LdsStack st{LdsMem{}, 16u * blockDim.x, lb + ts * blockDim.x + 16u * threadIdx.x};
+        write_qtab(LdsMem{}, tab, S, ts);
+        LdsStack st{LdsMem{}, 16u * kPlainBlock, lb + ts * kPlainBlock + 16u * threadIdx.x};
         const QTab qt{LdsMem{}, tab};
         uint64_t cnt, nn;
         run_root<NODES>(S, root[c], st, qt, cnt, nn);
@@ -106,7 +106,7 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
                          const uint32_t *__restrict__ order, const uint8_t *__restrict__ splits, uint32_t m,
                          uint32_t split_all,
                          uint32_t *__restrict__ nitems, const uint32_t *__restrict__ off,
-                         uint4 *__restrict__ items, uint32_t *__restrict__ item_c,
+                         uint4 *__restrict__ items,
                          unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
 {
     extern __shared__ uint4 lds[];
@@ -114,25 +114,31 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     if (q >= m) return;
     const uint32_t c = order ? order[q] : q;
     const Sched S = load_sched(sched, c);
+    const uint4 sa = sched[kSchedU4 * c], sb = sched[kSchedU4 * c + 1], si = sched[kSchedU4 * c + 2];
+    const uint4 meta = make_uint4(c, si.x, si.y, 0u);
     const uint32_t split = splits ? (uint32_t)splits[c] : split_all;
     const uint4 r = root[c];
     SlotStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     uint64_t cnt = 0, nn = 0;
     uint32_t n = 0;
-    const uint32_t base = WRITE ? off[q] : 0u;
+    uint4 *const out = WRITE ? items + (size_t)kItemU4 * off[q] : nullptr;
+    auto put = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t w) {
+        if (WRITE) {
+            uint4 *it = out + (size_t)kItemU4 * n;
+            it[0] = make_uint4(a, b, cc, w);
+            it[1] = meta;
+            it[2] = sa;
+            it[3] = sb;
+        }
+        n++;
+    };
     if (r.w != 0u) {
         if (S.T < 2u || split == 0u || split > S.T - 2u) {
             // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
-            if (WRITE) { items[base] = r; item_c[base] = c; }
-            n = 1;
+            put(r.x, r.y, r.z, r.w | (S.T ? entry(S, S.T - 1u) << 16 : 0u));
         } else {
-            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) {
-                if (WRITE) {
-                    items[base + n] = make_uint4(a, b, cc, f | (split << 19));
-                    item_c[base + n] = c;
-                }
-                n++;
-            };
+            const uint32_t up = entry(S, S.T - 1u - split) << 16;
+            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | up); };
             expand_subtree<NODES>(S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
         }
     }
@@ -173,19 +179,20 @@ template <bool LDS_STACK> struct BatchedStack;
 template <> struct BatchedStack<true> { using type = LdsStack; };
 template <> struct BatchedStack<false> { using type = PrivStack; };
 
-template <bool NODES>
-__device__ __forceinline__ void flush_counts(unsigned long long *raw, unsigned long long *nodes_out, uint32_t c,
-                                             uint32_t cnt, uint64_t nn)
-{
-    if (cnt) atomicAdd(raw + c, (unsigned long long)cnt);
-    if (NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);
-}
+// A lane counts the solutions of its current item in a u32 and hands it to
+// the record's u64 when the item ends or when the count reaches 2^31 (each
+// step adds at most one, so it never wraps).
+constexpr uint32_t kCountFlush = 1u << 31;
 
-template <bool NODES, bool LDS_STACK>
-__global__ void k_batched(const uint4 *__restrict__ items, const uint32_t *__restrict__ item_c,
-                          const uint4 *__restrict__ sched, const unsigned long long *__restrict__ nitems_total,
-                          unsigned int *__restrict__ head, uint32_t chunk, uint32_t ts,
-                          unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
+// Each lane walks one item and holds the next one, loaded while it walks:
+// when its walk ends the lane switches to the held item without waiting for
+// memory, then asks for a new one.  Items are handed out from a range the
+// wave claims `chunk` at a time from the global queue head.
+template <bool NODES, bool LDS_STACK, uint32_t BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict__ nitems_total,
+          unsigned int *__restrict__ head, uint32_t chunk, uint32_t ts,
+          unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
 {
     extern __shared__ uint4 lds[];
     const uint32_t lb = lds_addr(lds);
@@ -193,31 +200,47 @@ __global__ void k_batched(const uint4 *__restrict__ items, const uint32_t *__res
     const uint32_t lane = lane_id();
     const uint32_t tab = lb + threadIdx.x * ts;
     typename BatchedStack<LDS_STACK>::type st;
-    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * blockDim.x, lb + ts * blockDim.x + 16u * threadIdx.x};
-    QTab qt{LdsMem{}, tab};
+    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * BLOCK, lb + ts * BLOCK + 16u * threadIdx.x};
+    const QTab qt{LdsMem{}, tab};
     LaneK L = idle_lane();
     Walk w{0u, 0u, 0u, 0u, st.empty()};
     uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
     bool exhausted = false;           // wave-uniform: queue drained
-    bool have = false;                // this lane holds a subtree
+    bool have = false;                // this lane walks an item
+    bool held = false;                // this lane holds the next item (p0..p3)
+    uint4 p0{}, p1{}, p2{}, p3{};
     uint32_t c = 0, cnt = 0;
     uint64_t nn = 0;
     for (;;) {
         // every lane steps; idle lanes step without effect
-        bool wrap;
-        const bool fin = walk_step<NODES>(L, w, st, qt, cnt, nn, wrap) && have;
-        if (wave_ballot(fin || wrap || !have) == 0ull) continue;
-        if (wrap) atomicAdd(raw + c, 1ull << 32);
-        if (fin) {
-            flush_counts<NODES>(raw, nodes_out, c, cnt, nn);
-            have = false;
-            L = idle_lane();
-            w.W = 0u;
-            qt.off = tab;
+        const bool fin = walk_step<NODES>(L, w, st, qt, cnt, nn) && have;
+        if (wave_ballot(fin || !have || cnt >= kCountFlush) == 0ull) continue;
+        const uint32_t fc = c, fcnt = cnt;
+        const uint64_t fnn = nn;
+        const bool flush = fin || cnt >= kCountFlush;
+        if (flush) cnt = 0;
+        if (fin) { have = false; L = idle_lane(); w.W = 0u; }
+        // switch to the held item (its loads were issued at an earlier event)
+        if (!have && held) {
+            held = false;
+            c = p1.x;
+            const uint32_t T = p1.y & 0xFFu, N = (p1.y >> 16) & 31u;
+            nn = 0;
+            if (T == 0u) {
+                if (p0.w & p1.z) atomicAdd(raw + c, 1ull);   // a root on the terminal row
+            } else if ((p0.w & kAvailMask) != 0u) {
+                const uint32_t e[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; k++)
+                    if (4u * k < ts) LdsMem{}.st32(tab + 4u * k, e[k]);
+                L = LaneK{(1u << N) - 1u, N - 4u};
+                w = Walk{p0.x, p0.y, p0.z, p0.w, st.empty()};
+                have = true;
+            }
         }
-        // hand the next items of the wave's claimed range to the lanes without one
+        // hand out the next items of the wave's range to the lanes without a held item
         while (!exhausted) {
-            const uint64_t need = wave_ballot(!have);
+            const uint64_t need = wave_ballot(!held);
             if (need == 0ull) break;
             if (next >= end) {
                 uint32_t b = 0;
@@ -230,28 +253,17 @@ __global__ void k_batched(const uint4 *__restrict__ items, const uint32_t *__res
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
                                       __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
             const uint32_t idx = next + rank;
-            if (!have && idx < end) {
-                const uint4 it = items[idx];
-                c = item_c[idx];
-                const Sched S = load_sched(sched, c);
-                const uint32_t d0 = it.w >> 19, av = it.w & kAvailMask;
-                cnt = 0; nn = 0;
-                if (av != 0u) {
-                    if (S.T == 0u) {
-                        // a root on the terminal row
-                        if (av & S.tm) atomicAdd(raw + c, 1ull);
-                    } else {
-                        write_qtab(LdsMem{}, tab, S, d0, ts);
-                        qt.off = tab - d0;
-                        L = lane_consts(S);
-                        w = Walk{it.x, it.y, it.z, frame_word(av, d0, qbyte(S, d0)), st.empty()};
-                        have = true;
-                    }
-                }
+            if (!held && idx < end) {
+                const uint4 *it = items + (size_t)kItemU4 * idx;
+                p0 = it[0]; p1 = it[1]; p2 = it[2]; p3 = it[3];
+                held = true;
             }
             next = min(next + (uint32_t)__builtin_popcountll(need), end);
         }
-        if (wave_ballot(have) == 0ull) break;   // reached only once the queue is drained
+        // counts of the walks that ended (after the switch, which waits for older memory operations)
+        if (flush && fcnt) atomicAdd(raw + fc, (unsigned long long)fcnt);
+        if (NODES && fin && fnn) atomicAdd(nodes_out + fc, (unsigned long long)fnn);
+        if (wave_ballot(have || held) == 0ull) break;   // reached only once the queue is drained
     }
 }
 
@@ -262,7 +274,7 @@ __global__ void k_finalize(const uint4 *__restrict__ sched, uint32_t m, const un
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t v = 0;
     if (c < m) {
-        v = raw[c] << ((sched[2 * c + 1].y >> 8) & 3u);
+        v = raw[c] << ((sched[kSchedU4 * c + 2].x >> 8) & 3u);
         weighted[c] = v;
     }
     v = wave_sum_u64(v);
@@ -407,6 +419,7 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const
                         unsigned long long *total)
 {
     if (m == 0) return hipSuccess;
+    if (block != (int)kPlainBlock) return hipErrorInvalidValue;
     const size_t lds = lay.bytes(block);
     const dim3 grid((m + block - 1) / block);
     if (nodes) k_plain<true><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, lay.ts);
@@ -417,14 +430,14 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
                          const uint4 *root, const uint4 *sched, const uint32_t *order, const uint8_t *splits,
                          uint32_t m, uint32_t split,
-                         uint32_t *nitems, const uint32_t *off, uint4 *items, uint32_t *item_c,
+                         uint32_t *nitems, const uint32_t *off, uint4 *items,
                          unsigned long long *raw, unsigned long long *nodes_out)
 {
     if (m == 0) return hipSuccess;
     const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
     const dim3 grid((m + block - 1) / block);
 #define NQ_EXPAND(NB, WB) \
-    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, splits, m, split, nitems, off, items, item_c, raw, \
+    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, splits, m, split, nitems, off, items, raw, \
                                               nodes_out)
     if (write) { if (nodes) NQ_EXPAND(true, true); else NQ_EXPAND(false, true); }
     else { if (nodes) NQ_EXPAND(true, false); else NQ_EXPAND(false, false); }
@@ -438,25 +451,41 @@ hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32
     return hipGetLastError();
 }
 
+template <bool NB, bool LB>
+static const void *batched_fn(int block)
+{
+    switch (block) {
+    case 64: return (const void *)k_batched<NB, LB, 64>;
+    case 128: return (const void *)k_batched<NB, LB, 128>;
+    case 256: return (const void *)k_batched<NB, LB, 256>;
+    default: return nullptr;
+    }
+}
+
+static const void *batched_kernel(bool nodes, bool lds_stack, int block)
+{
+    if (lds_stack) return nodes ? batched_fn<true, true>(block) : batched_fn<false, true>(block);
+    return nodes ? batched_fn<true, false>(block) : batched_fn<false, false>(block);
+}
+
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,
-                          const uint4 *items, const uint32_t *item_c, const uint4 *sched,
+                          const uint4 *items,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,
                           unsigned long long *raw, unsigned long long *nodes_out)
 {
-    const size_t lds = lds_stack ? lay.bytes(block) : (size_t)block * lay.ts;
-#define NQ_BATCHED(NB, LB) \
-    k_batched<NB, LB><<<grid, block, lds, s>>>(items, item_c, sched, n_items, head, chunk, lay.ts, raw, nodes_out)
-    if (lds_stack) { if (nodes) NQ_BATCHED(true, true); else NQ_BATCHED(false, true); }
-    else { if (nodes) NQ_BATCHED(true, false); else NQ_BATCHED(false, false); }
-#undef NQ_BATCHED
-    return hipGetLastError();
+    const void *f = batched_kernel(nodes, lds_stack, block);
+    if (!f) return hipErrorInvalidValue;
+    const size_t lds = lds_stack ? lay.bytes(block) : (size_t)block * lay.ts + 32u;
+    uint32_t ts = lay.ts;
+    void *args[] = {(void *)&items, (void *)&n_items, (void *)&head, (void *)&chunk, (void *)&ts, (void *)&raw,
+                    (void *)&nodes_out};
+    return hipLaunchKernel(f, dim3(grid), dim3(block), args, lds, s);
 }
 
 hipError_t batched_occupancy(bool nodes, bool lds_stack, int block, size_t lds_bytes, int *blocks_per_cu)
 {
-    const void *f;
-    if (lds_stack) f = nodes ? (const void *)k_batched<true, true> : (const void *)k_batched<false, true>;
-    else f = nodes ? (const void *)k_batched<true, false> : (const void *)k_batched<false, false>;
+    const void *f = batched_kernel(nodes, lds_stack, block);
+    if (!f) return hipErrorInvalidValue;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, lds_bytes);
 }
 

@@ -20,7 +20,8 @@
 //   bit  6    NLA   no one-row lookahead prune at this depth (the prune runs
 //                   when availptn is set and row+1 < endmark)
 //   bit  7    JMN   the child is the jmark frame: column bit 0 is not one of
-//                   its candidates (18Py:419-421)
+//                   its candidates (18Py:419-421); also set on depth T-1 when
+//                   the terminal functionid is 14 (see below)
 //
 // T is the depth of the terminal row (row == endmark on a base functionid);
 // a child created at depth T counts one solution when its free set is
@@ -241,9 +242,32 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     if (T >= kMaxDepth - 1) { D.status = kErrDepth; return; }
     D.T = (uint32_t)T;
     D.tm = (f == 14) ? (mask & ~1u) : mask;
+    // functionid 14 counts a terminal child only for columns other than 0:
+    // the same as a jmark on the terminal row (which the row walk never
+    // produces itself), so the walk needs no separate terminal mask
+    if (f == 14 && T > 0) D.tab[(T - 1) >> 2] |= 0x80u << (((T - 1) & 3) * 8);
     // root frame, jmark applied when the root itself is the jmark frame
     if (jm[0]) { fr &= ~1u; ld |= 1u; }
     D.ld = ld; D.rd = rd; D.col = col; D.avail = fr;
+}
+
+// The per-record schedule the kernels read (3 x 16 bytes):
+//   words 0-7   16-bit entry r (r = 0..15) = the upper half of the frame word
+//               of a frame at depth T-1-r: (param byte << 3) | (r << 12);
+//               zero for r >= T
+//   word 8      T | wshift << 8 | N << 16
+//   word 9      terminal free mask (used for a root on the terminal row)
+//   word 10     decode status
+NQ_HD inline void sched_words(const Desc &D, int N, uint32_t w[12])
+{
+    for (int a = 0; a < 12; a++) w[a] = 0;
+    for (uint32_t r = 0; r < 16u && r < D.T; r++) {
+        const uint32_t e = (param_byte(D.tab, D.T - 1u - r) << 3) | (r << 12);
+        w[r >> 1] |= e << ((r & 1u) * 16u);
+    }
+    w[8] = D.T | (D.wshift << 8) | ((uint32_t)N << 16);
+    w[9] = D.tm;
+    w[10] = (uint32_t)D.status;
 }
 
 }  // namespace nqt
