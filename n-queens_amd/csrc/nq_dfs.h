@@ -16,39 +16,31 @@ namespace nqk {
 // per-record schedule (layout: nqt::sched_words)
 // ---------------------------------------------------------------------------
 struct Sched {
-    uint32_t e[8];            // 16-bit entries: frame-word upper half of depth T-1-r
+    uint32_t t[4];            // byte r: parameter byte of depth T-1-r
     uint32_t T;               // terminal depth (<= 16)
     uint32_t tm;              // terminal free mask
     uint32_t mask;            // board mask (1 << N) - 1
     uint32_t nsh;             // N - 4: rd block constants are ARc << nsh
 };
 
-NQ_HD inline uint32_t sched_word(const Sched &S, uint32_t i)
+// Byte r (< 16) of a 16-byte table held in four words.
+NQ_HD inline uint32_t table_byte(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3, uint32_t r)
 {
-    uint32_t v = S.e[0];
-#pragma unroll
-    for (uint32_t k = 1; k < 8u; k++) v = (i == k) ? S.e[k] : v;
-    return v;
-}
-
-// Entry r (r < 16): the upper half of the frame word of depth T-1-r.
-NQ_HD inline uint32_t entry(const Sched &S, uint32_t r)
-{
-    return (sched_word(S, r >> 1) >> ((r & 1u) * 16u)) & 0xFFFFu;
+    const uint32_t w = r < 4u ? t0 : r < 8u ? t1 : r < 12u ? t2 : t3;
+    return (w >> ((r & 3u) * 8u)) & 0xFFu;
 }
 
 // Parameter byte of depth d < T.
 NQ_HD inline uint32_t qbyte(const Sched &S, uint32_t d)
 {
-    return (entry(S, S.T - 1u - d) >> 3) & 0xFFu;
+    return table_byte(S.t[0], S.t[1], S.t[2], S.t[3], S.T - 1u - d);
 }
 
 NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 {
-    const uint4 a = sched[kSchedU4 * c], b = sched[kSchedU4 * c + 1], i = sched[kSchedU4 * c + 2];
+    const uint4 a = sched[kSchedU4 * c], i = sched[kSchedU4 * c + 1];
     Sched S;
-    S.e[0] = a.x; S.e[1] = a.y; S.e[2] = a.z; S.e[3] = a.w;
-    S.e[4] = b.x; S.e[5] = b.y; S.e[6] = b.z; S.e[7] = b.w;
+    S.t[0] = a.x; S.t[1] = a.y; S.t[2] = a.z; S.t[3] = a.w;
     S.T = i.x & 0xFFu;
     S.tm = i.y;
     const uint32_t N = (i.x >> 16) & 31u;
@@ -63,8 +55,7 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // Frame word: avail (bits 0-18) | parameter byte of the frame's depth
 // (19-26) | 0 (27) | r = T-1-depth (28-31).  The frame's children are on the
 // terminal row exactly when r = 0, i.e. when the word is below 2^28, and
-// (W - 2^28) >> 27 is twice the child's r: the byte offset of the child's
-// entry in the lane's table.
+// W - 2^28 holds the children's r in its top bits.
 constexpr uint32_t kAvailMask = 0x7FFFFu;
 constexpr uint32_t kQShift = 19u;
 constexpr uint32_t kRShift = 28u;
@@ -74,14 +65,39 @@ NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t r, uint32_t q)
     return avail | (q << kQShift) | (r << kRShift);
 }
 
-// Per-lane constants of the walk.  An idle lane (mask 0) steps without
-// effect: no candidate, no count, no stack motion.
+// Frame word of a frame at depth d with candidates avail.
+NQ_HD inline uint32_t start_word(const Sched &S, uint32_t avail, uint32_t d)
+{
+    return frame_word(avail, S.T - 1u - d, qbyte(S, d));
+}
+
+// Per-lane constants of the walk: board mask, rd shift and the parameter
+// bytes by r.  An idle lane (mask 0) steps without effect: no candidate, no
+// count, no stack motion.
 struct LaneK {
     uint32_t mask;
     uint32_t nsh;
+    uint32_t t0, t1, t2, t3;
 };
-NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.nsh}; }
-NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u}; }
+NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.nsh, S.t[0], S.t[1], S.t[2], S.t[3]}; }
+NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, 0u, 0u, 0u, 0u}; }
+
+// Parameter byte of the children of a frame, from Wd = W - 2^28 (top bits:
+// the children's r).  On the device two v_perm_b32 pick the byte from each
+// register pair (selector bytes 1-3 = 0x0c give zeros) and the sign of Wd
+// (r >= 8) picks the pair; an indexed select over the words would be
+// lowered to a scratch array.
+NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t Wd)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t sel = ((Wd >> kRShift) & 7u) | 0x0c0c0c00u;
+    const uint32_t a = __builtin_amdgcn_perm(L.t1, L.t0, sel);
+    const uint32_t b = __builtin_amdgcn_perm(L.t3, L.t2, sel);
+    return (int32_t)Wd < 0 ? b : a;
+#else
+    return table_byte(L.t0, L.t1, L.t2, L.t3, Wd >> kRShift);
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // LDS access by 32-bit LDS address (on the device the address goes straight
@@ -134,23 +150,6 @@ struct PrivStack {
     NQ_HD inline void write(uint32_t o, uint4 v) { a[o >> 4] = v; }
 };
 
-// A lane's frame-word upper halves indexed by r: entry r (at byte 2r) is
-// the child frame at depth T-1-r.
-struct QTab {
-    LdsMem m;
-    uint32_t off;
-    NQ_HD inline uint32_t at(uint32_t byte_off) const { return m.ld2(off + byte_off); }
-};
-
-// Write a lane's first ts/4 schedule words (ts a multiple of 4, <= 32) at
-// LDS address a.
-NQ_HD inline void write_qtab(const LdsMem &m, uint32_t a, const Sched &S, uint32_t ts)
-{
-#pragma unroll
-    for (uint32_t k = 0; k < 8u; k++)
-        if (4u * k < ts) m.st32(a + 4u * k, S.e[k]);
-}
-
 // A lane's DFS position: the current frame and the address of the top
 // stack entry.
 struct Walk {
@@ -169,7 +168,7 @@ struct Walk {
 // with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots.
 // Returns true when the walk's subtree is exhausted.
 template <bool NODES, class Count, class Stack>
-NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, Count &cnt, uint64_t &nodes)
+NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint64_t &nodes)
 {
     const uint32_t S = st.S;
     const uint4 top = st.read((uint32_t)max((int32_t)w.A, (int32_t)st.base));
@@ -184,8 +183,8 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, 
     const uint32_t nf = L.mask & ~(occ | ncol);
     const uint32_t nla = (uint32_t)((int32_t)(W << 6) >> 31);   // all ones when the frame has no prune
     const bool lok = ((L.mask & ~((nld << 1) | (nrd >> 1) | ncol)) | nla) != 0u;
-    const uint32_t Wd = W - (1u << kRShift);             // child's r in the top bits
-    const uint32_t hc = qt.at(Wd >> 27);                 // child frame word, upper half
+    const uint32_t Wd = W - (1u << kRShift);             // children's r in the top bits
+    const uint32_t qc = child_q(L, Wd);
     const bool inner = W >= (1u << kRShift);             // children not on the terminal row
     const bool ok = lok & (nf != 0u);
     const bool down = ok & inner;
@@ -199,7 +198,7 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, const QTab &qt, 
     const bool push = down & more;
     const bool pop = !down & !more & (w.A != st.empty());
     if (push) st.write(w.A + S, make_uint4(w.ld, w.rd, w.col, rest));
-    const uint32_t Wc = (hc << 16) + nf;              // disjoint fields
+    const uint32_t Wc = (Wd & (15u << kRShift)) | (qc << kQShift) | nf;
     w.ld = down ? nld : (pop ? top.x : w.ld);
     w.rd = down ? nrd : (pop ? top.y : w.rd);
     w.col = down ? ncol : (pop ? top.z : w.col);

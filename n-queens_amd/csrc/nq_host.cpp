@@ -250,8 +250,8 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
         NQ_HIPCHK(hipMemcpy(sc.data(), c->sched.p, nqk::kSchedU4 * m * 16, hipMemcpyDeviceToHost));
         c->Tmin = 0xFFu;
         for (uint64_t a = 0; a < m; a++) {
-            c->Tmax = std::max(c->Tmax, sc[nqk::kSchedU4 * a + 2].x & 0xFFu);
-            c->Tmin = std::min(c->Tmin, sc[nqk::kSchedU4 * a + 2].x & 0xFFu);
+            c->Tmax = std::max(c->Tmax, sc[nqk::kSchedU4 * a + 1].x & 0xFFu);
+            c->Tmin = std::min(c->Tmin, sc[nqk::kSchedU4 * a + 1].x & 0xFFu);
         }
     }
     // the walk keeps 16 parameter bytes per record (depths 0..15), so the
@@ -340,7 +340,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     const int max_split = c->Tmax >= 2 ? (int)c->Tmax - 2 : 0;
     if (s.split_depth > max_split) s.split_depth = max_split;
     const nqk::WalkLds lay = batched_lds(c, s.split_depth);
-    const size_t lds_bytes = s.lds_stack ? lay.bytes(s.block) : (size_t)s.block * lay.ts;
+    const size_t lds_bytes = s.lds_stack ? lay.bytes(s.block) : 0u;
     if (lds_bytes > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, c->lds_max);
     if (c->m == 0) {
         if (total_out) *total_out = 0;

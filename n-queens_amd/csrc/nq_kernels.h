@@ -6,24 +6,20 @@
 namespace nqk {
 
 constexpr int kStackCap = 16;  // private-memory DFS stack entries (>= T-1, T <= 16)
-constexpr uint32_t kSchedU4 = 3;   // uint4 words of a record's schedule (nqt::sched_words)
-// A work item, 4 x uint4: {ld, rd, col, frame word}, {record, T | wshift << 8 |
-// N << 16, terminal mask, 0}, and the record's schedule words 0-7.
-constexpr uint32_t kItemU4 = 4;
+constexpr uint32_t kSchedU4 = 2;   // uint4 words of a record's schedule (nqt::sched_words)
+// A work item, 3 x uint4: {ld, rd, col, frame word}, {record, T | wshift << 8 |
+// N << 16, terminal mask, 0}, and the record's parameter bytes (schedule words 0-3).
+constexpr uint32_t kItemU4 = 3;
 
-// LDS of a walking workgroup: `ts` table bytes per lane (16-bit entries of
-// the children's frame-word upper halves by r = T-1-depth; a walk from depth
-// d0 uses r < T-1-d0, and its reads for terminal children land at byte 30,
-// inside the LDS that follows), then the stack, `slots` entries of 16 bytes
-// per lane (at most T-1-d0 frames are pushed).
+// LDS of a walking workgroup: the DFS stacks, `slots` entries of 16 bytes per
+// lane (a walk from depth d0 pushes at most T-1-d0 frames).
 struct WalkLds {
-    uint32_t ts, slots;
-    size_t bytes(int block) const { return (size_t)block * (ts + 16u * slots); }
+    uint32_t slots;
+    size_t bytes(int block) const { return (size_t)block * 16u * slots; }
 };
 inline WalkLds walk_lds(uint32_t Tmax, uint32_t d0)
 {
-    const uint32_t entries = Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u;
-    return WalkLds{4u * ((entries + 1u) / 2u), entries};
+    return WalkLds{Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u};
 }
 
 hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,

@@ -27,15 +27,14 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Count + nodes of a whole constellation from its root frame (depth 0).
 template <bool NODES, class Stack>
-__device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, const QTab &qt, uint64_t &cnt,
-                                         uint64_t &nodes)
+__device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, uint64_t &cnt, uint64_t &nodes)
 {
     cnt = 0; nodes = 0;
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
-    Walk w{root.x, root.y, root.z, root.w | (entry(S, S.T - 1u) << 16), st.empty()};
-    while (!walk_step<NODES>(L, w, st, qt, cnt, nodes)) {
+    Walk w{root.x, root.y, root.z, start_word(S, root.w, 0u), st.empty()};
+    while (!walk_step<NODES>(L, w, st, cnt, nodes)) {
     }
 }
 
@@ -56,11 +55,10 @@ __global__ void k_decode(int N, const uint8_t *__restrict__ Ns, const uint4 *__r
     nqt::decode(n, r.x, r.y, r.z, r.w, D);
     if (D.status != nqt::kOk) atomicOr(err, 1u << D.status);
     root[c] = make_uint4(D.ld, D.rd, D.col, D.avail);
-    uint32_t w[12];
+    uint32_t w[8];
     nqt::sched_words(D, n, w);
     sched[kSchedU4 * c] = make_uint4(w[0], w[1], w[2], w[3]);
     sched[kSchedU4 * c + 1] = make_uint4(w[4], w[5], w[6], w[7]);
-    sched[kSchedU4 * c + 2] = make_uint4(w[8], w[9], w[10], w[11]);
 }
 
 // ---------------------------------------------------------------------------
@@ -72,7 +70,7 @@ template <bool NODES>
 __global__ void __launch_bounds__(kPlainBlock)
 k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_t m,
         unsigned long long *__restrict__ weighted, unsigned long long *__restrict__ nodes_out,
-        unsigned long long *__restrict__ total, uint32_t ts)
+        unsigned long long *__restrict__ total)
 {
     extern __shared__ uint4 lds[];
     const uint32_t lb = lds_addr(lds);
@@ -80,13 +78,10 @@ k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_
     uint64_t wsum = 0;
     if (c < m) {
         const Sched S = load_sched(sched, c);
-        const uint32_t wsh = (sched[kSchedU4 * c + 2].x >> 8) & 3u;
-        const uint32_t tab = lb + threadIdx.x * ts;
-        write_qtab(LdsMem{}, tab, S, ts);
-        LdsStack st{LdsMem{}, 16u * kPlainBlock, lb + ts * kPlainBlock + 16u * threadIdx.x};
-        const QTab qt{LdsMem{}, tab};
+        const uint32_t wsh = (sched[kSchedU4 * c + 1].x >> 8) & 3u;
+        LdsStack st{LdsMem{}, 16u * kPlainBlock, lb + 16u * threadIdx.x};
         uint64_t cnt, nn;
-        run_root<NODES>(S, root[c], st, qt, cnt, nn);
+        run_root<NODES>(S, root[c], st, cnt, nn);
         wsum = cnt << wsh;
         weighted[c] = wsum;
         if (NODES) nodes_out[c] = nn;
@@ -114,7 +109,7 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     if (q >= m) return;
     const uint32_t c = order ? order[q] : q;
     const Sched S = load_sched(sched, c);
-    const uint4 sa = sched[kSchedU4 * c], sb = sched[kSchedU4 * c + 1], si = sched[kSchedU4 * c + 2];
+    const uint4 sa = sched[kSchedU4 * c], si = sched[kSchedU4 * c + 1];
     const uint4 meta = make_uint4(c, si.x, si.y, 0u);
     const uint32_t split = splits ? (uint32_t)splits[c] : split_all;
     const uint4 r = root[c];
@@ -128,16 +123,15 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
             it[0] = make_uint4(a, b, cc, w);
             it[1] = meta;
             it[2] = sa;
-            it[3] = sb;
         }
         n++;
     };
     if (r.w != 0u) {
         if (S.T < 2u || split == 0u || split > S.T - 2u) {
             // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
-            put(r.x, r.y, r.z, r.w | (S.T ? entry(S, S.T - 1u) << 16 : 0u));
+            put(r.x, r.y, r.z, S.T ? start_word(S, r.w, 0u) : r.w);
         } else {
-            const uint32_t up = entry(S, S.T - 1u - split) << 16;
+            const uint32_t up = start_word(S, 0u, split);
             auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | up); };
             expand_subtree<NODES>(S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
         }
@@ -191,29 +185,27 @@ constexpr uint32_t kCountFlush = 1u << 31;
 template <bool NODES, bool LDS_STACK, uint32_t BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict__ nitems_total,
-          unsigned int *__restrict__ head, uint32_t chunk, uint32_t ts,
+          unsigned int *__restrict__ head, uint32_t chunk,
           unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
 {
     extern __shared__ uint4 lds[];
     const uint32_t lb = lds_addr(lds);
     const uint32_t n_items = (uint32_t)*nitems_total;
     const uint32_t lane = lane_id();
-    const uint32_t tab = lb + threadIdx.x * ts;
     typename BatchedStack<LDS_STACK>::type st;
-    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * BLOCK, lb + ts * BLOCK + 16u * threadIdx.x};
-    const QTab qt{LdsMem{}, tab};
+    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * BLOCK, lb + 16u * threadIdx.x};
     LaneK L = idle_lane();
     Walk w{0u, 0u, 0u, 0u, st.empty()};
     uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
     bool exhausted = false;           // wave-uniform: queue drained
     bool have = false;                // this lane walks an item
     bool held = false;                // this lane holds the next item (p0..p3)
-    uint4 p0{}, p1{}, p2{}, p3{};
+    uint4 p0{}, p1{}, p2{};
     uint32_t c = 0, cnt = 0;
     uint64_t nn = 0;
     for (;;) {
         // every lane steps; idle lanes step without effect
-        const bool fin = walk_step<NODES>(L, w, st, qt, cnt, nn) && have;
+        const bool fin = walk_step<NODES>(L, w, st, cnt, nn) && have;
         if (wave_ballot(fin || !have || cnt >= kCountFlush) == 0ull) continue;
         const uint32_t fc = c, fcnt = cnt;
         const uint64_t fnn = nn;
@@ -229,11 +221,7 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
             if (T == 0u) {
                 if (p0.w & p1.z) atomicAdd(raw + c, 1ull);   // a root on the terminal row
             } else if ((p0.w & kAvailMask) != 0u) {
-                const uint32_t e[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
-#pragma unroll
-                for (uint32_t k = 0; k < 8u; k++)
-                    if (4u * k < ts) LdsMem{}.st32(tab + 4u * k, e[k]);
-                L = LaneK{(1u << N) - 1u, N - 4u};
+                L = LaneK{(1u << N) - 1u, N - 4u, p2.x, p2.y, p2.z, p2.w};
                 w = Walk{p0.x, p0.y, p0.z, p0.w, st.empty()};
                 have = true;
             }
@@ -255,7 +243,7 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
             const uint32_t idx = next + rank;
             if (!held && idx < end) {
                 const uint4 *it = items + (size_t)kItemU4 * idx;
-                p0 = it[0]; p1 = it[1]; p2 = it[2]; p3 = it[3];
+                p0 = it[0]; p1 = it[1]; p2 = it[2];
                 held = true;
             }
             next = min(next + (uint32_t)__builtin_popcountll(need), end);
@@ -274,7 +262,7 @@ __global__ void k_finalize(const uint4 *__restrict__ sched, uint32_t m, const un
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t v = 0;
     if (c < m) {
-        v = raw[c] << ((sched[kSchedU4 * c + 2].x >> 8) & 3u);
+        v = raw[c] << ((sched[kSchedU4 * c + 1].x >> 8) & 3u);
         weighted[c] = v;
     }
     v = wave_sum_u64(v);
@@ -422,8 +410,8 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const
     if (block != (int)kPlainBlock) return hipErrorInvalidValue;
     const size_t lds = lay.bytes(block);
     const dim3 grid((m + block - 1) / block);
-    if (nodes) k_plain<true><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, lay.ts);
-    else k_plain<false><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total, lay.ts);
+    if (nodes) k_plain<true><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total);
+    else k_plain<false><<<grid, block, lds, s>>>(root, sched, m, weighted, nodes_out, total);
     return hipGetLastError();
 }
 
@@ -475,9 +463,8 @@ hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, i
 {
     const void *f = batched_kernel(nodes, lds_stack, block);
     if (!f) return hipErrorInvalidValue;
-    const size_t lds = lds_stack ? lay.bytes(block) : (size_t)block * lay.ts + 32u;
-    uint32_t ts = lay.ts;
-    void *args[] = {(void *)&items, (void *)&n_items, (void *)&head, (void *)&chunk, (void *)&ts, (void *)&raw,
+    const size_t lds = lds_stack ? lay.bytes(block) : 0u;
+    void *args[] = {(void *)&items, (void *)&n_items, (void *)&head, (void *)&chunk, (void *)&raw,
                     (void *)&nodes_out};
     return hipLaunchKernel(f, dim3(grid), dim3(block), args, lds, s);
 }

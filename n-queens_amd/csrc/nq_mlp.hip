@@ -28,7 +28,7 @@ __global__ void k_features(const uint4 *__restrict__ rec, const uint4 *__restric
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m) return;
     const uint4 r = rec[c];
-    const int N = (int)((sched[kSchedU4 * c + 2].x >> 16) & 31u);
+    const int N = (int)((sched[kSchedU4 * c + 1].x >> 16) & 31u);
     nqt::Desc D;
     nqt::decode(N, r.x, r.y, r.z, r.w, D);
     const uint4 fr = root[c];

@@ -251,23 +251,21 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     D.ld = ld; D.rd = rd; D.col = col; D.avail = fr;
 }
 
-// The per-record schedule the kernels read (3 x 16 bytes):
-//   words 0-7   16-bit entry r (r = 0..15) = the upper half of the frame word
-//               of a frame at depth T-1-r: (param byte << 3) | (r << 12);
-//               zero for r >= T
-//   word 8      T | wshift << 8 | N << 16
-//   word 9      terminal free mask (used for a root on the terminal row)
-//   word 10     decode status
-NQ_HD inline void sched_words(const Desc &D, int N, uint32_t w[12])
+// The per-record schedule the kernels read (2 x 16 bytes):
+//   words 0-3   byte r (r = 0..15) = parameter byte of depth T-1-r, the byte
+//               a frame whose children are r rows above the terminal row
+//               gives its child; zero for r >= T
+//   word 4      T | wshift << 8 | N << 16
+//   word 5      terminal free mask (used for a root on the terminal row)
+//   word 6      decode status
+NQ_HD inline void sched_words(const Desc &D, int N, uint32_t w[8])
 {
-    for (int a = 0; a < 12; a++) w[a] = 0;
-    for (uint32_t r = 0; r < 16u && r < D.T; r++) {
-        const uint32_t e = (param_byte(D.tab, D.T - 1u - r) << 3) | (r << 12);
-        w[r >> 1] |= e << ((r & 1u) * 16u);
-    }
-    w[8] = D.T | (D.wshift << 8) | ((uint32_t)N << 16);
-    w[9] = D.tm;
-    w[10] = (uint32_t)D.status;
+    for (int a = 0; a < 8; a++) w[a] = 0;
+    for (uint32_t r = 0; r < 16u && r < D.T; r++)
+        w[r >> 2] |= param_byte(D.tab, D.T - 1u - r) << ((r & 3u) * 8u);
+    w[4] = D.T | (D.wshift << 8) | ((uint32_t)N << 16);
+    w[5] = D.tm;
+    w[6] = (uint32_t)D.status;
 }
 
 }  // namespace nqt
