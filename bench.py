@@ -14,7 +14,9 @@ One node = one candidate extraction of the reference dfs_iter
 (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:368-562); node counts come
 from a node-counting run of the same kernel (checked against the oracle by
 the parity tests).  Roofline: 12 algorithmic int32 ops per node against the
-INT32 VALU peak of 256 CUs x 64 lanes x 2.4 GHz.
+INT32 VALU peak of 256 CUs x 128 lanes x 2.4 GHz = 78.6 T ops/s (a CU issues
+two wave64 32-bit VALU instructions per clock: the published 157.3 TFLOP/s
+FP32 vector peak is the same 128 lanes counting an FMA as two flops).
 
 Prints one JSON line.
 """
@@ -36,7 +38,7 @@ for _p in (ROOT / "n-queens_amd", ROOT):
 
 METRIC = "N=19/21 exact-count wall time & DFS nodes/s: batched+cost-model vs plain kernel"
 OPS_PER_NODE = 12
-PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12   # 39.32 T int32 VALU ops/s
+PEAK_INT32_TOPS = 256 * 128 * 2.4e9 / 1e12   # 78.64 T int32 VALU ops/s
 
 
 def log(msg: str) -> None:
