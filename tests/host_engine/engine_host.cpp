@@ -1,0 +1,90 @@
+// Host build of the device DFS engine (n-queens_amd/csrc/nq_task.h, nq_dfs.h):
+// the work-item expansion and the walk of every item, record by record, with
+// the LDS stack emulated in a byte array.  Used by tests/test_engine_host.py
+// to compare the kernels' logic with the oracle on the CPU; not product code.
+#include "../../n-queens_amd/csrc/nq_task.h"
+#include "../../n-queens_amd/csrc/nq_dfs.h"
+#include <stdint.h>
+#include <vector>
+
+using namespace nqk;
+
+namespace {
+
+uint8_t g_lds[1 << 16];
+
+// Count one item (a frame at depth d0); -1 on a stack overflow past the
+// slots the kernels allocate.
+int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32_t av, uint32_t d0,
+                  uint64_t &nodes, bool priv)
+{
+    if (S.T == 0) return (av & S.tm) ? 1 : 0;
+    const WalkLds lay = walk_lds(S.T, d0);
+    const uint32_t lane = 5, block = 64;   // lane 5 of a 64-lane workgroup
+    LdsMem m{g_lds};
+    for (size_t a = 0; a < lay.bytes(block) + 64; a++) g_lds[a] = 0xA5;   // poison
+    const LaneK L = lane_consts(S);
+    uint32_t cnt = 0;
+    uint64_t total = 0;
+    bool done;
+    if (priv) {
+        PrivStack st;
+        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
+        do {
+            done = walk_step<true>(L, w, st, cnt, nodes);
+            if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
+        } while (!done);
+    } else {
+        LdsStack st{m, 16u * block, 16u * lane};
+        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
+        do {
+            done = walk_step<true>(L, w, st, cnt, nodes);
+            if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
+            if (w.A != st.empty() && w.A >= st.base + lay.slots * st.S) return -1;
+        } while (!done);
+    }
+    return (int64_t)total;
+}
+
+}  // namespace
+
+// Weighted count and node count of every record, expanded at `split` (0 =
+// whole records) and walked with the LDS (priv = 0) or private stack.
+// Returns 0, -1 - decode status, or -100 on a stack overflow.
+extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes, int split,
+                            int priv)
+{
+    for (long a = 0; a < m; a++) {
+        nqt::Desc D;
+        const uint32_t *r = rec + 4 * a;
+        nqt::decode(N, r[0], r[1], r[2], r[3], D);
+        if (D.status) return -1 - D.status;
+        uint32_t wd[8];
+        nqt::sched_words(D, N, wd);
+        const uint4 sc[2] = {make_uint4(wd[0], wd[1], wd[2], wd[3]), make_uint4(wd[4], wd[5], wd[6], wd[7])};
+        const Sched S = load_sched(sc, 0);
+        uint64_t c = 0, n = 0;
+        if (split > 0 && D.T >= 2 && (uint32_t)split <= D.T - 2 && D.avail) {
+            struct It { uint32_t a, b, c, f; };
+            std::vector<It> items;
+            uint64_t c0 = 0;
+            auto emit = [&](uint32_t x, uint32_t y, uint32_t z, uint32_t f) { items.push_back({x, y, z, f}); };
+            uint4 slots[kStackCap];
+            SlotStack st{slots, 0};
+            expand_subtree<true>(S, D.ld, D.rd, D.col, D.avail, 0u, (uint32_t)split, st, c0, n, emit);
+            c = c0;
+            for (const It &it : items) {
+                const int64_t k = walk_item(S, it.a, it.b, it.c, it.f, (uint32_t)split, n, priv != 0);
+                if (k < 0) return -100;
+                c += (uint64_t)k;
+            }
+        } else if (D.avail) {
+            const int64_t k = walk_item(S, D.ld, D.rd, D.col, D.avail, 0u, n, priv != 0);
+            if (k < 0) return -100;
+            c = (uint64_t)k;
+        }
+        weighted[a] = c << D.wshift;
+        nodes[a] = n;
+    }
+    return 0;
+}

@@ -1,0 +1,74 @@
+"""The device DFS engine (csrc/nq_task.h + nq_dfs.h: decode, schedule words,
+expansion, walk_step with both stacks) built for the host and compared with
+the oracle record by record: weighted counts and node counts, every split
+depth, LDS and private stacks, and the stack-slot bound the kernels allocate."""
+import ctypes
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from nqueens import constellations as C
+
+ROOT = Path(__file__).resolve().parent.parent
+SRC = ROOT / "tests" / "host_engine" / "engine_host.cpp"
+
+
+def _hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    return None
+
+
+@pytest.fixture(scope="module")
+def engine(tmp_path_factory):
+    hipcc = _hipcc()
+    if hipcc is None:
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("engine") / "engine_host.so"
+    cmd = [hipcc, "--cuda-host-only", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", str(out), str(SRC)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-2000:]
+    lib = ctypes.CDLL(str(out))
+    u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+    lib.engine_count.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u64p, u64p, ctypes.c_int, ctypes.c_int]
+    lib.engine_count.restype = ctypes.c_int
+    return lib
+
+
+def _run(engine, N, rec, split, priv):
+    rec = np.ascontiguousarray(rec, dtype=np.uint32)
+    w = np.zeros(len(rec), np.uint64)
+    n = np.zeros(len(rec), np.uint64)
+    rc = engine.engine_count(N, rec, len(rec), w, n, split, priv)
+    return rc, w, n
+
+
+@pytest.mark.parametrize("N,preset", [(N, p) for N in range(7, 13) for p in (4, 5, 6)] + [(13, 5), (14, 4)])
+def test_engine_matches_oracle(engine, oracle, N, preset):
+    rec = C.gen_constellations(N, preset)
+    w, _, nodes = oracle.count_records(N, rec)
+    for split in (0, 1, 2, 3):
+        for priv in (0, 1):
+            rc, gw, gn = _run(engine, N, rec, split, priv)
+            assert rc == 0, (split, priv, rc)
+            assert np.array_equal(gw, w), (split, priv)
+            assert np.array_equal(gn, nodes), (split, priv)
+
+
+@pytest.mark.parametrize("N", [20, 21])
+def test_engine_largest_boards_sample(engine, N):
+    # records of the N=20/21 oracle fixture (terminal depth 14-15, long ld
+    # histories: the lookahead's occupancy word fills all 32 bits)
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "oracle_sample_n20_n21.npz") as z:
+        k = 12 if N == 20 else 40
+        rec, w, nodes = z[f"rec{N}"][::k], z[f"w{N}"][::k], z[f"nodes{N}"][::k]
+    rc, gw, gn = _run(engine, N, rec, 3, 0)
+    assert rc == 0
+    assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
