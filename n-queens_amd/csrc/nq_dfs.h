@@ -130,24 +130,30 @@ NQ_HD inline uint32_t lds_addr(const void *) { return 0u; }
 
 // ---------------------------------------------------------------------------
 // stacks of 16-byte frames {ld, rd, col, frame word}, byte-addressed:
-// slot i of a lane at base + i*S.  The walk keeps the address of its top
-// entry, which is base - S for an empty stack.
+// slot i of a lane at base + i*S.  The slot below base holds the inert frame
+// {0, 0, all columns, no candidates}: the top entry of an empty stack.  The
+// walk keeps the address of its top entry (base - S when empty); a walk that
+// ends takes the inert frame and stays on it, stepping without effect.
 // ---------------------------------------------------------------------------
+NQ_HD inline uint4 inert_frame() { return make_uint4(0u, 0u, ~0u, 0u); }
+
 struct LdsStack {
     LdsMem m;
     uint32_t S;       // slot stride: 16 bytes x workgroup size
-    uint32_t base;    // this lane's slot 0
+    uint32_t base;    // this lane's slot 0 (the inert frame sits at base - S)
     NQ_HD inline uint32_t empty() const { return base - S; }
     NQ_HD inline uint4 read(uint32_t a) const { return m.ld4(a); }
     NQ_HD inline void write(uint32_t a, uint4 v) { m.st4(a, v); }
+    NQ_HD inline void init() { write(empty(), inert_frame()); }
 };
 struct PrivStack {
-    uint4 a[kStackCap];
+    uint4 a[kStackCap + 1];
     uint32_t S = 16u;
-    uint32_t base = 0u;
+    uint32_t base = 16u;
     NQ_HD inline uint32_t empty() const { return base - S; }
     NQ_HD inline uint4 read(uint32_t o) const { return a[o >> 4]; }
     NQ_HD inline void write(uint32_t o, uint4 v) { a[o >> 4] = v; }
+    NQ_HD inline void init() { write(empty(), inert_frame()); }
 };
 
 // A lane's DFS position: the current frame and the address of the top
@@ -165,13 +171,14 @@ struct Walk {
 // The lane then descends (pushing the frame's remainder when it has one),
 // keeps the remainder, or pops.  The top entry is read every step (used only
 // when popping), so its load is issued a full step before its use.  Frames
-// with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots.
-// Returns true when the walk's subtree is exhausted.
+// with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots
+// above the inert frame's.  Returns true when the walk's subtree is
+// exhausted (the step took the inert frame).
 template <bool NODES, class Count, class Stack>
 NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint64_t &nodes)
 {
     const uint32_t S = st.S;
-    const uint4 top = st.read((uint32_t)max((int32_t)w.A, (int32_t)st.base));
+    const uint4 top = st.read(w.A);
     const uint32_t W = w.W;
     const uint32_t bit = W & (0u - W);                   // lowest candidate (avail is the low field)
     const uint32_t rest = W ^ bit;
@@ -193,18 +200,20 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 #if defined(__HIP_DEVICE_COMPILE__)
     asm volatile("" : "+v"(cnt));   // keeps the add beside its condition (one add-with-carry)
 #endif
-    if (NODES) nodes += 1u;
+    if (NODES) nodes += (uint64_t)(W != 0u);             // the inert frame is no node
     const bool more = (rest & kAvailMask) != 0u;
     const bool push = down & more;
-    const bool pop = !down & !more & (w.A != st.empty());
+    const bool up = !down & !more;                       // take the top entry
+    const bool nonempty = w.A != st.empty();
+    const bool pop = up & nonempty;                      // ... and drop it from the stack
     if (push) st.write(w.A + S, make_uint4(w.ld, w.rd, w.col, rest));
     const uint32_t Wc = (Wd & (15u << kRShift)) | (qc << kQShift) | nf;
-    w.ld = down ? nld : (pop ? top.x : w.ld);
-    w.rd = down ? nrd : (pop ? top.y : w.rd);
-    w.col = down ? ncol : (pop ? top.z : w.col);
-    w.W = down ? Wc : (pop ? top.w : rest);
+    w.ld = down ? nld : (up ? top.x : w.ld);
+    w.rd = down ? nrd : (up ? top.y : w.rd);
+    w.col = down ? ncol : (up ? top.z : w.col);
+    w.W = down ? Wc : (up ? top.w : rest);
     w.A += push ? S : (pop ? 0u - S : 0u);
-    return !down & !more & !pop;
+    return up & !nonempty;
 }
 
 // ---------------------------------------------------------------------------

@@ -12,14 +12,15 @@ constexpr uint32_t kSchedU4 = 2;   // uint4 words of a record's schedule (nqt::s
 constexpr uint32_t kItemU4 = 3;
 
 // LDS of a walking workgroup: the DFS stacks, `slots` entries of 16 bytes per
-// lane (a walk from depth d0 pushes at most T-1-d0 frames).
+// lane: the inert frame and the T-1-d0 frames a walk from depth d0 pushes at
+// most.
 struct WalkLds {
     uint32_t slots;
     size_t bytes(int block) const { return (size_t)block * 16u * slots; }
 };
 inline WalkLds walk_lds(uint32_t Tmax, uint32_t d0)
 {
-    return WalkLds{Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u};
+    return WalkLds{1u + (Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u)};
 }
 
 hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,

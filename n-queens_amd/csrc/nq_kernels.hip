@@ -79,7 +79,8 @@ k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_
     if (c < m) {
         const Sched S = load_sched(sched, c);
         const uint32_t wsh = (sched[kSchedU4 * c + 1].x >> 8) & 3u;
-        LdsStack st{LdsMem{}, 16u * kPlainBlock, lb + 16u * threadIdx.x};
+        LdsStack st{LdsMem{}, 16u * kPlainBlock, lb + 16u * (kPlainBlock + threadIdx.x)};
+        st.init();
         uint64_t cnt, nn;
         run_root<NODES>(S, root[c], st, cnt, nn);
         wsum = cnt << wsh;
@@ -178,6 +179,11 @@ template <> struct BatchedStack<false> { using type = PrivStack; };
 // step adds at most one, so it never wraps).
 constexpr uint32_t kCountFlush = 1u << 31;
 
+// Steps between two looks at the wave's lanes: a lane whose walk ends sits
+// on the inert frame and waits at most this many steps for its next item;
+// the steps in between carry no wave-wide check.
+constexpr int kStepsPerCheck = 8;
+
 // Each lane walks one item and holds the next one, loaded while it walks:
 // when its walk ends the lane switches to the held item without waiting for
 // memory, then asks for a new one.  Items are handed out from a range the
@@ -193,31 +199,35 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
     const uint32_t n_items = (uint32_t)*nitems_total;
     const uint32_t lane = lane_id();
     typename BatchedStack<LDS_STACK>::type st;
-    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * BLOCK, lb + 16u * threadIdx.x};
+    if constexpr (LDS_STACK) st = LdsStack{LdsMem{}, 16u * BLOCK, lb + 16u * (BLOCK + threadIdx.x)};
+    st.init();
     LaneK L = idle_lane();
-    Walk w{0u, 0u, 0u, 0u, st.empty()};
+    const uint4 f0 = inert_frame();
+    Walk w{f0.x, f0.y, f0.z, f0.w, st.empty()};
     uint32_t next = 0, end = 0;       // wave-uniform claimed range [next, end)
     bool exhausted = false;           // wave-uniform: queue drained
     bool have = false;                // this lane walks an item
-    bool held = false;                // this lane holds the next item (p0..p3)
+    bool held = false;                // this lane holds the next item (p0..p2)
+    bool pend = false;                // this lane's ended walk still has counts to hand in
     uint4 p0{}, p1{}, p2{};
-    uint32_t c = 0, cnt = 0;
-    uint64_t nn = 0;
+    uint32_t c = 0, cnt = 0, fc = 0, fcnt = 0;
+    uint64_t nn = 0, fnn = 0;
     for (;;) {
-        // every lane steps; idle lanes step without effect
-        const bool fin = walk_step<NODES>(L, w, st, cnt, nn) && have;
-        if (wave_ballot(fin || !have || cnt >= kCountFlush) == 0ull) continue;
-        const uint32_t fc = c, fcnt = cnt;
-        const uint64_t fnn = nn;
-        const bool flush = fin || cnt >= kCountFlush;
-        if (flush) cnt = 0;
-        if (fin) { have = false; L = idle_lane(); w.W = 0u; }
-        // switch to the held item (its loads were issued at an earlier event)
+#pragma unroll
+        for (int u = 0; u < kStepsPerCheck; u++) (void)walk_step<NODES>(L, w, st, cnt, nn);
+        const bool inert = w.W == 0u;   // a walking frame always has candidates
+        if (wave_ballot(inert || cnt >= kCountFlush) == 0ull) continue;
+        if (have && inert) {
+            // the walk ended: hand in its counts after the switch below
+            fc = c; fcnt = cnt; fnn = nn; pend = true;
+            cnt = 0; have = false;
+        }
+        // switch to the held item (its loads were issued at an earlier check)
         if (!have && held) {
             held = false;
             c = p1.x;
             const uint32_t T = p1.y & 0xFFu, N = (p1.y >> 16) & 31u;
-            nn = 0;
+            cnt = 0; nn = 0;
             if (T == 0u) {
                 if (p0.w & p1.z) atomicAdd(raw + c, 1ull);   // a root on the terminal row
             } else if ((p0.w & kAvailMask) != 0u) {
@@ -248,9 +258,13 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
             }
             next = min(next + (uint32_t)__builtin_popcountll(need), end);
         }
-        // counts of the walks that ended (after the switch, which waits for older memory operations)
-        if (flush && fcnt) atomicAdd(raw + fc, (unsigned long long)fcnt);
-        if (NODES && fin && fnn) atomicAdd(nodes_out + fc, (unsigned long long)fnn);
+        // counts (after the switch, which waits for older memory operations)
+        if (pend) {
+            if (fcnt) atomicAdd(raw + fc, (unsigned long long)fcnt);
+            if (NODES && fnn) atomicAdd(nodes_out + fc, (unsigned long long)fnn);
+            pend = false;
+        }
+        if (have && cnt >= kCountFlush) { atomicAdd(raw + c, (unsigned long long)cnt); cnt = 0; }
         if (wave_ballot(have || held) == 0ull) break;   // reached only once the queue is drained
     }
 }

@@ -14,7 +14,7 @@ namespace {
 uint8_t g_lds[1 << 16];
 
 // Count one item (a frame at depth d0); -1 on a stack overflow past the
-// slots the kernels allocate.
+// slots the kernels allocate, -2 if the ended walk does not stay inert.
 int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32_t av, uint32_t d0,
                   uint64_t &nodes, bool priv)
 {
@@ -29,19 +29,26 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     bool done;
     if (priv) {
         PrivStack st;
+        st.init();
         Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
         } while (!done);
     } else {
-        LdsStack st{m, 16u * block, 16u * lane};
+        LdsStack st{m, 16u * block, 16u * (block + lane)};
+        st.init();
         Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
-            if (w.A != st.empty() && w.A >= st.base + lay.slots * st.S) return -1;
+            if (w.A >= st.empty() + lay.slots * st.S) return -1;   // past the slots the kernels allocate
         } while (!done);
+        // the walk sits on the inert frame: further steps change nothing
+        const Walk w0 = w;
+        const uint64_t n0 = nodes;
+        for (int k = 0; k < 3; k++) (void)walk_step<true>(L, w, st, cnt, nodes);
+        if (cnt || nodes != n0 || w.W != w0.W || w.A != w0.A || w.col != w0.col) return -2;
     }
     return (int64_t)total;
 }
@@ -50,7 +57,8 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
 
 // Weighted count and node count of every record, expanded at `split` (0 =
 // whole records) and walked with the LDS (priv = 0) or private stack.
-// Returns 0, -1 - decode status, or -100 on a stack overflow.
+// Returns 0, -1 - decode status, -101 on a stack overflow, -102 if an ended
+// walk does not stay inert.
 extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes, int split,
                             int priv)
 {
@@ -75,12 +83,12 @@ extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weight
             c = c0;
             for (const It &it : items) {
                 const int64_t k = walk_item(S, it.a, it.b, it.c, it.f, (uint32_t)split, n, priv != 0);
-                if (k < 0) return -100;
+                if (k < 0) return (int)(-100 + k);
                 c += (uint64_t)k;
             }
         } else if (D.avail) {
             const int64_t k = walk_item(S, D.ld, D.rd, D.col, D.avail, 0u, n, priv != 0);
-            if (k < 0) return -100;
+            if (k < 0) return (int)(-100 + k);
             c = (uint64_t)k;
         }
         weighted[a] = c << D.wshift;
