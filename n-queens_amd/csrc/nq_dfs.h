@@ -105,14 +105,10 @@ NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t Wd)
 // ---------------------------------------------------------------------------
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
-typedef __attribute__((address_space(3))) uint16_t lds_u16;
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint4 lds_u4;
 struct LdsMem {
     NQ_HD inline uint4 ld4(uint32_t a) const { return *(const lds_u4 *)(size_t)a; }
     NQ_HD inline void st4(uint32_t a, uint4 v) const { *(lds_u4 *)(size_t)a = v; }
-    NQ_HD inline uint32_t ld2(uint32_t a) const { return *(const lds_u16 *)(size_t)a; }
-    NQ_HD inline void st32(uint32_t a, uint32_t v) const { *(lds_u32 *)(size_t)a = v; }
 };
 // LDS address of a pointer into the workgroup's shared memory
 __device__ inline uint32_t lds_addr(const void *p) { return (uint32_t)(size_t)(const lds_u8 *)p; }
@@ -121,8 +117,6 @@ struct LdsMem {
     uint8_t *mem;
     NQ_HD inline uint4 ld4(uint32_t a) const { return *(const uint4 *)(mem + a); }
     NQ_HD inline void st4(uint32_t a, uint4 v) const { *(uint4 *)(mem + a) = v; }
-    NQ_HD inline uint32_t ld2(uint32_t a) const { return *(const uint16_t *)(mem + a); }
-    NQ_HD inline void st32(uint32_t a, uint32_t v) const { *(uint32_t *)(mem + a) = v; }
 };
 // for the kernels' host pass only (kernels never run on the host)
 NQ_HD inline uint32_t lds_addr(const void *) { return 0u; }
