@@ -200,13 +200,20 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const bool up = !down & !more;                       // take the top entry
     const bool nonempty = w.A != st.empty();
     const bool pop = up & nonempty;                      // ... and drop it from the stack
-    if (push) st.write(w.A + S, make_uint4(w.ld, w.rd, w.col, rest));
+    const uint4 rem = make_uint4(w.ld, w.rd, w.col, rest);
+    const uint32_t A = w.A;
     const uint32_t Wc = (Wd & (15u << kRShift)) | (qc << kQShift) | nf;
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
     w.col = down ? ncol : (up ? top.z : w.col);
     w.W = down ? Wc : (up ? top.w : rest);
     w.A += push ? S : (pop ? 0u - S : 0u);
+    // written after the selects: the wait for the top entry does not also
+    // wait for this write
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::"v"(w.ld), "v"(w.rd), "v"(w.col), "v"(w.W));
+#endif
+    if (push) st.write(A + S, rem);
     return up & !nonempty;
 }
 

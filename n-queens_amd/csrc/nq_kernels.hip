@@ -182,7 +182,7 @@ constexpr uint32_t kCountFlush = 1u << 31;
 // Steps between two looks at the wave's lanes: a lane whose walk ends sits
 // on the inert frame and waits at most this many steps for its next item;
 // the steps in between carry no wave-wide check.
-constexpr int kStepsPerCheck = 8;
+constexpr int kStepsPerCheck = 16;
 
 // Each lane walks one item and holds the next one, loaded while it walks:
 // when its walk ends the lane switches to the held item without waiting for
