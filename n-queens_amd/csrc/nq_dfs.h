@@ -52,24 +52,27 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // ---------------------------------------------------------------------------
 // frames
 // ---------------------------------------------------------------------------
-// Frame word: avail (bits 0-18) | parameter byte of the frame's depth
-// (19-26) | 0 (27) | r = T-1-depth (28-31).  The frame's children are on the
-// terminal row exactly when r = 0, i.e. when the word is below 2^28, and
-// W - 2^28 holds the children's r in its top bits.
-constexpr uint32_t kAvailMask = 0x7FFFFu;
-constexpr uint32_t kQShift = 19u;
+// A frame is {ld, rd, col, W}.  The frame word W holds the candidates
+// (avail, bits 0-27) and r = T-1-depth (bits 28-31): the frame's children are
+// on the terminal row exactly when r = 0, i.e. when W is below 2^28, and
+// W - 2^28 holds the children's r in its top bits.  col holds the occupied
+// columns in bits 0-23 (N <= 24) and the parameter byte of the frame's depth
+// in bits 24-31; the board mask drops that byte wherever col meets a board
+// set.
+constexpr uint32_t kAvailMask = 0x0FFFFFFFu;
+constexpr uint32_t kQShift = 24u;
 constexpr uint32_t kRShift = 28u;
+constexpr uint32_t kColMask = 0x00FFFFFFu;
 
-NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t r, uint32_t q)
-{
-    return avail | (q << kQShift) | (r << kRShift);
-}
+NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t r) { return avail | (r << kRShift); }
+NQ_HD inline uint32_t frame_col(uint32_t col, uint32_t q) { return col | (q << kQShift); }
 
-// Frame word of a frame at depth d with candidates avail.
+// Frame word and col of a frame at depth d with candidates avail.
 NQ_HD inline uint32_t start_word(const Sched &S, uint32_t avail, uint32_t d)
 {
-    return frame_word(avail, S.T - 1u - d, qbyte(S, d));
+    return frame_word(avail, S.T - 1u - d);
 }
+NQ_HD inline uint32_t start_col(const Sched &S, uint32_t col, uint32_t d) { return frame_col(col, qbyte(S, d)); }
 
 // Per-lane constants of the walk: board mask, rd shift and the parameter
 // bytes by r.  An idle lane (mask 0) steps without effect: no candidate, no
@@ -82,20 +85,21 @@ struct LaneK {
 NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.nsh, S.t[0], S.t[1], S.t[2], S.t[3]}; }
 NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, 0u, 0u, 0u, 0u}; }
 
-// Parameter byte of the children of a frame, from Wd = W - 2^28 (top bits:
-// the children's r).  On the device two v_perm_b32 pick the byte from each
-// register pair (selector bytes 1-3 = 0x0c give zeros) and the sign of Wd
-// (r >= 8) picks the pair; an indexed select over the words would be
-// lowered to a scratch array.
+// Parameter byte of the children of a frame, in bits 24-31 (bits 0-23 are
+// not defined), from Wd = W - 2^28 (top bits: the children's r).  On the
+// device two v_perm_b32 place the byte from each register pair in byte 3:
+// the selector is Wd's bits 4-30, whose byte 3 is r & 7 (its other bytes
+// select arbitrary bytes); the sign of Wd (r >= 8) picks the pair.  An
+// indexed select over the words would be lowered to a scratch array.
 NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t Wd)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint32_t sel = ((Wd >> kRShift) & 7u) | 0x0c0c0c00u;
+    const uint32_t sel = __builtin_amdgcn_ubfe(Wd, 4u, 27u);
     const uint32_t a = __builtin_amdgcn_perm(L.t1, L.t0, sel);
     const uint32_t b = __builtin_amdgcn_perm(L.t3, L.t2, sel);
     return (int32_t)Wd < 0 ? b : a;
 #else
-    return table_byte(L.t0, L.t1, L.t2, L.t3, Wd >> kRShift);
+    return table_byte(L.t0, L.t1, L.t2, L.t3, Wd >> kRShift) << kQShift;
 #endif
 }
 
@@ -173,16 +177,19 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 {
     const uint32_t S = st.S;
     const uint4 top = st.read(w.A);
-    const uint32_t W = w.W;
+    const uint32_t W = w.W, C = w.col;
     const uint32_t bit = W & (0u - W);                   // lowest candidate (avail is the low field)
     const uint32_t rest = W ^ bit;
-    const uint32_t s = (W >> 19) & 3u;
-    const uint32_t nld = ((w.ld | bit) << s) | ((W >> 21) & 3u);
-    const uint32_t nrd = ((w.rd | bit) >> s) | (((W >> 23) & 3u) << L.nsh);
-    const uint32_t ncol = w.col | bit;
-    const uint32_t occ = nld | nrd | ((W >> 26) & 1u);   // JMN: column 0 of a jmark child
-    const uint32_t nf = L.mask & ~(occ | ncol);
-    const uint32_t nla = (uint32_t)((int32_t)(W << 6) >> 31);   // all ones when the frame has no prune
+    const uint32_t s = (C >> 24) & 3u;
+    const uint32_t nld = ((w.ld | bit) << s) | ((C >> 26) & 3u);
+    const uint32_t nrd = ((w.rd | bit) >> s) | (((C >> 28) & 3u) << L.nsh);
+    const uint32_t ncol = C | bit;
+    const uint32_t occ = nld | nrd | (C >> 31);          // JMN: column 0 of a jmark child
+    uint32_t nf = L.mask & ~(occ | ncol);
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
+#endif
+    const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);   // all ones when the frame has no prune
     const bool lok = ((L.mask & ~((nld << 1) | (nrd >> 1) | ncol)) | nla) != 0u;
     const uint32_t Wd = W - (1u << kRShift);             // children's r in the top bits
     const uint32_t qc = child_q(L, Wd);
@@ -200,12 +207,13 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const bool up = !down & !more;                       // take the top entry
     const bool nonempty = w.A != st.empty();
     const bool pop = up & nonempty;                      // ... and drop it from the stack
-    const uint4 rem = make_uint4(w.ld, w.rd, w.col, rest);
+    const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
     const uint32_t A = w.A;
-    const uint32_t Wc = (Wd & (15u << kRShift)) | (qc << kQShift) | nf;
+    const uint32_t Wc = (Wd & (15u << kRShift)) | nf;
+    const uint32_t colc = (ncol & kColMask) | (qc & ~kColMask);
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
-    w.col = down ? ncol : (up ? top.z : w.col);
+    w.col = down ? colc : (up ? top.z : C);
     w.W = down ? Wc : (up ? top.w : rest);
     w.A += push ? S : (pop ? 0u - S : 0u);
     // written after the selects: the wait for the top entry does not also
@@ -265,7 +273,7 @@ NQ_HD inline void expand_subtree(const Sched &S, uint32_t ld, uint32_t rd, uint3
             sp--;
             const uint4 e = st.get(sp);
             ld = e.x; rd = e.y; col = e.z;
-            avail = e.w & kAvailMask;
+            avail = e.w & kColMask;   // avail < 2^22 sits below the depth byte
             d = e.w >> 24;
             q = qbyte(S, d);
         }

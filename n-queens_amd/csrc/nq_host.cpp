@@ -87,7 +87,8 @@ int size_items(nq_ctx *c, int split, int block)
     unsigned long long cnt = 0;
     NQ_HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     NQ_HIPCHK(hipStreamSynchronize(c->stream));
-    if (cnt >= (1ull << 32)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^32)", split, cnt);
+    // item indices, the queue head and its claims past the end are u32
+    if (cnt >= (1ull << 31)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^31)", split, cnt);
     NQ_HIPCHK(c->items.reserve(nqk::kItemU4 * cnt));
     c->item_total = cnt;
     c->sized_split = split;
@@ -186,13 +187,14 @@ namespace {
 int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_t m, const uint32_t *order)
 {
     if (int r = check_ctx(c)) return r;
-    if (!Ns && (N < nqt::kMinN || N > nqt::kMaxN)) return nq_fail(NQ_EINVAL, "N=%d outside the u32 path (6..21)", N);
+    if (!Ns && (N < nqt::kMinN || N > nqt::kMaxN)) return nq_fail(NQ_EINVAL, "N=%d outside the u32 path (%d..%d)", N, nqt::kMinN, nqt::kMaxN);
     std::vector<uint8_t> n8;
     if (Ns) {
         n8.resize(m);
         for (uint64_t a = 0; a < m; a++) {
             if (Ns[a] < nqt::kMinN || Ns[a] > nqt::kMaxN)
-                return nq_fail(NQ_EINVAL, "record %llu: N=%d outside the u32 path (6..21)", (unsigned long long)a, Ns[a]);
+                return nq_fail(NQ_EINVAL, "record %llu: N=%d outside the u32 path (%d..%d)", (unsigned long long)a, Ns[a],
+                               nqt::kMinN, nqt::kMaxN);
             n8[a] = (uint8_t)Ns[a];
         }
     }
@@ -438,7 +440,7 @@ int nq_symmetric_counts(nq_ctx *c, int N, uint64_t *f90_out, uint64_t *f180_out)
 
 int nq_decode_host(int N, const uint32_t *rec4, uint64_t m, int32_t *out10)
 {
-    if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside 6..21", N);
+    if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside %d..%d", N, nqt::kMinN, nqt::kMaxN);
     if (m && (!rec4 || !out10)) return nq_fail(NQ_EINVAL, "null pointer");
     for (uint64_t a = 0; a < m; a++) {
         nqt::Desc D;

@@ -4,7 +4,7 @@
 //
 // Work unit: one DFS node is one extraction bit = avail & -avail of the
 // reference dfs_iter (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:368-562).
-// All masks are u32 (N <= 21); counts are u64.
+// All masks are u32 (N <= 23); counts are u64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "nq_task.h"
@@ -33,7 +33,7 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
-    Walk w{root.x, root.y, root.z, start_word(S, root.w, 0u), st.empty()};
+    Walk w{root.x, root.y, start_col(S, root.z, 0u), start_word(S, root.w, 0u), st.empty()};
     while (!walk_step<NODES>(L, w, st, cnt, nodes)) {
     }
 }
@@ -130,10 +130,11 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     if (r.w != 0u) {
         if (S.T < 2u || split == 0u || split > S.T - 2u) {
             // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
-            put(r.x, r.y, r.z, S.T ? start_word(S, r.w, 0u) : r.w);
+            if (S.T) put(r.x, r.y, start_col(S, r.z, 0u), start_word(S, r.w, 0u));
+            else put(r.x, r.y, r.z, r.w);
         } else {
-            const uint32_t up = start_word(S, 0u, split);
-            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | up); };
+            const uint32_t upw = start_word(S, 0u, split), upc = start_col(S, 0u, split);
+            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc | upc, f | upw); };
             expand_subtree<NODES>(S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
         }
     }

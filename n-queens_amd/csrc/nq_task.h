@@ -37,11 +37,14 @@ namespace nqt {
 
 constexpr int kMaxDepth = 24;        // parameter bytes per descriptor (6 words)
 constexpr int kMinN = 6;
-constexpr int kMaxN = 21;            // u32 masks: avail < 2^19, rd < 2^31
+// u32 masks: the board and col fit 24 bits (col's top byte carries the
+// walk's parameter byte); build_soa's rd bit (N-1-j) + (N-2-start) is set
+// only when j >= 2N-33-start, which keeps it at bit 30 or below
+constexpr int kMaxN = 23;
 
 enum : int32_t {
     kOk = 0,
-    kErrP5 = 1,          // target functionid 8..11 (never produced for N <= 21)
+    kErrP5 = 1,          // target functionid 8..11 (not produced for N <= 23 at presets 5-6)
     kErrShift = 2,       // a shift of build_soa_for_range out of the u32 range
     kErrNoTerminal = 3,  // row sequence passes endmark without a base frame
     kErrDepth = 4,       // terminal depth beyond kMaxDepth

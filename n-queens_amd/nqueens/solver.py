@@ -26,7 +26,7 @@ from ._lib import Context, Sched, as_records
 __all__ = ["count", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
            "preset_for", "MIN_N", "MAX_N"]
 
-MIN_N, MAX_N = 6, 21
+MIN_N, MAX_N = 6, 23
 
 _ctx: dict[int, Context] = {}
 _ctx_lock = threading.Lock()
@@ -43,8 +43,11 @@ def default_context(device: int = 0) -> Context:
 
 
 def preset_for(N: int) -> int:
-    """Preset-queen depth: 18Py always uses 5 (:2139)."""
-    return 5
+    """Preset-queen depth: 5, as 18Py always uses (:2139), up to N = 22.
+
+    The walk keeps 16 parameter bytes per record, and a record of preset p
+    has its terminal row at depth N-1-p, so N = 23 starts from preset 6."""
+    return max(5, N - 17)
 
 
 def default_sched(N: int | None = None) -> Sched:
@@ -102,7 +105,7 @@ def exec_solutions(N: int, constellations: list, use_gpu: bool = True) -> None:
 
 def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
           sched: Sched | None = None, ctx: Context | None = None) -> tuple[int, int]:
-    """Total and unique N-Queens solutions for 6 <= N <= 21, counted on the GPU."""
+    """Total and unique N-Queens solutions for 6 <= N <= 23, counted on the GPU."""
     _check_n(N)
     ctx = ctx or default_context()
     rec = _cons.gen_constellations(N, preset or preset_for(N))

@@ -30,7 +30,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     if (priv) {
         PrivStack st;
         st.init();
-        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
+        Walk w{ld, rd, start_col(S, col, d0), start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
@@ -38,7 +38,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     } else {
         LdsStack st{m, 16u * block, 16u * (block + lane)};
         st.init();
-        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
+        Walk w{ld, rd, start_col(S, col, d0), start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }

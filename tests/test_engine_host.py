@@ -72,3 +72,18 @@ def test_engine_largest_boards_sample(engine, N):
     rc, gw, gn = _run(engine, N, rec, 3, 0)
     assert rc == 0
     assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+
+
+@pytest.mark.parametrize("N", [22, 23])
+def test_engine_n22_n23_sample(engine, N):
+    # N=22 (preset 5, terminal depth 16, 20-bit candidate sets) and N=23
+    # (preset 6, terminal depth 16, 21-bit candidate sets): the widest boards
+    # the u32 frame layout holds
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "oracle_sample_n22_n23.npz") as z:
+        k = 32 if N == 22 else 52
+        rec, w, nodes = z[f"rec{N}"][::k], z[f"w{N}"][::k], z[f"nodes{N}"][::k]
+    split, priv = (3, 0) if N == 22 else (5, 1)
+    rc, gw, gn = _run(engine, N, rec, split, priv)
+    assert rc == 0
+    assert np.array_equal(gw, w) and np.array_equal(gn, nodes)

@@ -105,7 +105,7 @@ def test_batched_fixture_digest(ctx, digests, N):
     assert solutions_sha256(rec, gw) == digests[str(N)]["solutions_sha256"]
 
 
-@pytest.mark.parametrize("N", range(4, 22))
+@pytest.mark.parametrize("N", range(4, 24))
 def test_symmetric_counts(ctx, oracle, N):
     assert ctx.symmetric_counts(N) == oracle.symmetric_counts(N)
 
@@ -128,7 +128,7 @@ def test_empty_set_and_errors(ctx):
     assert ctx.run_plain()[0] == 0
     assert ctx.run_batched(sched())[0] == 0
     with pytest.raises(nqueens.NQError):
-        ctx.load(22, C.gen_constellations(12, 5))
+        ctx.load(24, C.gen_constellations(12, 5))
     # a P5 target (functionid 8..11) is refused, never counted
     bad = np.array([[0, 0, 0, (2 << 20) | C.to_ijkl(5, 3, 7, 9)]], np.uint32)
     assert nqueens.decode_host(21, bad)[0, 9] == 1
@@ -179,31 +179,37 @@ def test_per_record_split_depths(ctx, oracle):
 
 
 @pytest.fixture(scope="module")
-def sample_n20_n21():
+def oracle_samples():
     from tests.conftest import GOLDEN
-    with np.load(GOLDEN / "oracle_sample_n20_n21.npz") as z:
-        return {k: z[k] for k in z.files}
+    out = {}
+    for name in ("oracle_sample_n20_n21.npz", "oracle_sample_n22_n23.npz"):
+        with np.load(GOLDEN / name) as z:
+            out.update({k: z[k] for k in z.files})
+    return out
 
 
-# The largest boards: every 400th record of N=20/21 (cached oracle counts).
-# Long ld histories fill all 32 bits of the lookahead's occupancy word here,
-# which N <= 19 rarely does; the prune switch must not depend on a spare bit.
-@pytest.mark.parametrize("N", [20, 21])
-@pytest.mark.parametrize("mode", ["plain", "batched"])
-def test_largest_n_oracle_sample(ctx, sample_n20_n21, N, mode):
-    rec = sample_n20_n21[f"rec{N}"]
+# The largest boards: every 400th record of N=20/21 (preset 5), every
+# 2,000th of N=22 (preset 5) and every 20,000th of N=23 (preset 6), with
+# cached oracle counts.  Long ld histories fill all 32 bits of the
+# lookahead's occupancy word here, which N <= 19 rarely does; N=22/23 use
+# terminal depth 16 and 20/21-bit candidate sets.  The plain kernel (one lane
+# per record) is left out at N=22/23, where one record is up to 4e8 nodes.
+@pytest.mark.parametrize("N,mode", [(20, "plain"), (20, "batched"), (21, "plain"), (21, "batched"),
+                                    (22, "batched"), (23, "batched")])
+def test_largest_n_oracle_sample(ctx, oracle_samples, N, mode):
+    rec = oracle_samples[f"rec{N}"]
     ctx.load(N, rec)
     if mode == "plain":
         total, _ = ctx.run_plain(count_nodes=True)
     else:
         total, _ = ctx.run_batched(sched(3, 256, 1), count_nodes=True)
     gw, gn = ctx.results(weighted=True, nodes=True)
-    assert np.array_equal(gw, sample_n20_n21[f"w{N}"])
-    assert np.array_equal(gn, sample_n20_n21[f"nodes{N}"])
-    assert total == int(sample_n20_n21[f"w{N}"].sum())
+    assert np.array_equal(gw, oracle_samples[f"w{N}"])
+    assert np.array_equal(gn, oracle_samples[f"nodes{N}"])
+    assert total == int(oracle_samples[f"w{N}"].sum())
 
 
-@pytest.mark.parametrize("N", [20, 21])
+@pytest.mark.parametrize("N", [20, 21, 22])
 def test_largest_n_full_count(ctx, known, N):
     total, unique = nqueens.count(N, ctx=ctx)
     assert [total, unique] == list(known["readme_total_unique"][str(N)])
