@@ -16,7 +16,8 @@ namespace nqk {
 // per-record schedule (layout: nqt::sched_words)
 // ---------------------------------------------------------------------------
 struct Sched {
-    uint32_t t[4];            // byte r: parameter byte of depth T-1-r
+    uint32_t t[4];            // byte r (1..15): parameter byte of depth T-r
+    uint32_t q0;              // parameter byte of depth 0
     uint32_t T;               // terminal depth (<= 16)
     uint32_t tm;              // terminal free mask
     uint32_t mask;            // board mask (1 << N) - 1
@@ -33,7 +34,7 @@ NQ_HD inline uint32_t table_byte(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t
 // Parameter byte of depth d < T.
 NQ_HD inline uint32_t qbyte(const Sched &S, uint32_t d)
 {
-    return table_byte(S.t[0], S.t[1], S.t[2], S.t[3], S.T - 1u - d);
+    return d == 0u ? S.q0 : table_byte(S.t[0], S.t[1], S.t[2], S.t[3], S.T - d);
 }
 
 NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
@@ -42,6 +43,7 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
     Sched S;
     S.t[0] = a.x; S.t[1] = a.y; S.t[2] = a.z; S.t[3] = a.w;
     S.T = i.x & 0xFFu;
+    S.q0 = i.x >> 24;
     S.tm = i.y;
     const uint32_t N = (i.x >> 16) & 31u;
     S.mask = (1u << N) - 1u;
@@ -54,8 +56,8 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // ---------------------------------------------------------------------------
 // A frame is {ld, rd, col, W}.  The frame word W holds the candidates
 // (avail, bits 0-27) and r = T-1-depth (bits 28-31): the frame's children are
-// on the terminal row exactly when r = 0, i.e. when W is below 2^28, and
-// W - 2^28 holds the children's r in its top bits.  col holds the occupied
+// on the terminal row exactly when r = 0, i.e. when W is below 2^28, and a
+// child's word is W's top bits less 2^28 plus its candidates.  col holds the occupied
 // columns in bits 0-23 (N <= 24) and the parameter byte of the frame's depth
 // in bits 24-31; the board mask drops that byte wherever col meets a board
 // set.
@@ -86,20 +88,20 @@ NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.nsh, S.t
 NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, 0u, 0u, 0u, 0u}; }
 
 // Parameter byte of the children of a frame, in bits 24-31 (bits 0-23 are
-// not defined), from Wd = W - 2^28 (top bits: the children's r).  On the
+// not defined), from the frame word (top bits: the frame's r).  On the
 // device two v_perm_b32 place the byte from each register pair in byte 3:
-// the selector is Wd's bits 4-30, whose byte 3 is r & 7 (its other bytes
-// select arbitrary bytes); the sign of Wd (r >= 8) picks the pair.  An
+// the selector is W's bits 4-30, whose byte 3 is r & 7 (its other bytes
+// select arbitrary bytes); the sign of W (r >= 8) picks the pair.  An
 // indexed select over the words would be lowered to a scratch array.
-NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t Wd)
+NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t W)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint32_t sel = __builtin_amdgcn_ubfe(Wd, 4u, 27u);
+    const uint32_t sel = __builtin_amdgcn_ubfe(W, 4u, 27u);
     const uint32_t a = __builtin_amdgcn_perm(L.t1, L.t0, sel);
     const uint32_t b = __builtin_amdgcn_perm(L.t3, L.t2, sel);
-    return (int32_t)Wd < 0 ? b : a;
+    return (int32_t)W < 0 ? b : a;
 #else
-    return table_byte(L.t0, L.t1, L.t2, L.t3, Wd >> kRShift) << kQShift;
+    return table_byte(L.t0, L.t1, L.t2, L.t3, W >> kRShift) << kQShift;
 #endif
 }
 
@@ -191,8 +193,8 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 #endif
     const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);   // all ones when the frame has no prune
     const bool lok = ((L.mask & ~((nld << 1) | (nrd >> 1) | ncol)) | nla) != 0u;
-    const uint32_t Wd = W - (1u << kRShift);             // children's r in the top bits
-    const uint32_t qc = child_q(L, Wd);
+    const uint32_t hiW = W & ~kAvailMask;                // r << 28
+    const uint32_t qc = child_q(L, W);
     const bool inner = W >= (1u << kRShift);             // children not on the terminal row
     const bool ok = lok & (nf != 0u);
     const bool down = ok & inner;
@@ -202,14 +204,14 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     asm volatile("" : "+v"(cnt));   // keeps the add beside its condition (one add-with-carry)
 #endif
     if (NODES) nodes += (uint64_t)(W != 0u);             // the inert frame is no node
-    const bool more = (rest & kAvailMask) != 0u;
+    const bool more = rest > hiW;                        // candidates left after this one
     const bool push = down & more;
     const bool up = !down & !more;                       // take the top entry
     const bool nonempty = w.A != st.empty();
     const bool pop = up & nonempty;                      // ... and drop it from the stack
     const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
     const uint32_t A = w.A;
-    const uint32_t Wc = (Wd & (15u << kRShift)) | nf;
+    const uint32_t Wc = hiW + nf - (1u << kRShift);      // the child: r - 1, candidates nf
     const uint32_t colc = (ncol & kColMask) | (qc & ~kColMask);
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);

@@ -255,18 +255,19 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
 }
 
 // The per-record schedule the kernels read (2 x 16 bytes):
-//   words 0-3   byte r (r = 0..15) = parameter byte of depth T-1-r, the byte
-//               a frame whose children are r rows above the terminal row
-//               gives its child; zero for r >= T
-//   word 4      T | wshift << 8 | N << 16
+//   words 0-3   byte r (r = 1..15) = parameter byte of depth T-r: the byte a
+//               frame with r = T-1-depth gives its children; byte 0 (the
+//               children of a frame at depth T-1 are terminal) and bytes
+//               r >= T are zero
+//   word 4      T | wshift << 8 | N << 16 | parameter byte of depth 0 << 24
 //   word 5      terminal free mask (used for a root on the terminal row)
 //   word 6      decode status
 NQ_HD inline void sched_words(const Desc &D, int N, uint32_t w[8])
 {
     for (int a = 0; a < 8; a++) w[a] = 0;
-    for (uint32_t r = 0; r < 16u && r < D.T; r++)
-        w[r >> 2] |= param_byte(D.tab, D.T - 1u - r) << ((r & 3u) * 8u);
-    w[4] = D.T | (D.wshift << 8) | ((uint32_t)N << 16);
+    for (uint32_t r = 1; r < 16u && r < D.T; r++)
+        w[r >> 2] |= param_byte(D.tab, D.T - r) << ((r & 3u) * 8u);
+    w[4] = D.T | (D.wshift << 8) | ((uint32_t)N << 16) | (param_byte(D.tab, 0) << 24);
     w[5] = D.tm;
     w[6] = (uint32_t)D.status;
 }
