@@ -76,16 +76,56 @@ NQ_HD inline uint32_t start_word(const Sched &S, uint32_t avail, uint32_t d)
 }
 NQ_HD inline uint32_t start_col(const Sched &S, uint32_t col, uint32_t d) { return frame_col(col, qbyte(S, d)); }
 
-// Per-lane constants of the walk: board mask, rd shift and the parameter
-// bytes by r.  An idle lane (mask 0) steps without effect: no candidate, no
-// count, no stack motion.
+// Any function of three words, bit by bit: bit i of IMM is the result for
+// the input bits (a, b, c) = (i >> 2, i >> 1, i) & 1.  On gfx950 this is one
+// v_bitop3_b32, which issues at the full VALU rate with three VGPR operands
+// (v_or3_b32, v_and_or_b32, v_bfi_b32, v_perm_b32 and v_cndmask_b32 issue at
+// half rate).
+template <uint32_t IMM>
+NQ_HD inline uint32_t bop3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, IMM);
+#else
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < 8u; i++)
+        if ((IMM >> i) & 1u) r |= ((i & 4u) ? a : ~a) & ((i & 2u) ? b : ~b) & ((i & 1u) ? c : ~c);
+    return r;
+#endif
+}
+constexpr uint32_t kOr3 = 0xFEu;        // a | b | c
+constexpr uint32_t kSel = 0xCAu;        // a ? b : c, bit by bit
+constexpr uint32_t kAndNotOr = 0x10u;   // a & ~(b | c)
+constexpr uint32_t kAndNotOrC = 0xBAu;  // (a & ~b) | c
+constexpr uint32_t kOrAnd = 0xF8u;      // a | (b & c)
+
+// A constant held in a VGPR: bitop3 reads its operands from VGPRs at the
+// full rate, an SGPR or literal operand halves it.
+NQ_HD inline uint32_t vconst(uint32_t v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
+
+// Per-lane constants of the walk: board mask, rd shift, the parameter bytes
+// by r and the column field mask of col (in a VGPR).  An idle lane (mask 0)
+// steps without effect: no candidate, no count, no stack motion.
+// The rd block constant ARc << (N-4) comes straight from col: ARc sits in
+// col's bits 28-29, so (col >> (32-N)) & (3 << (N-4)) is it.
 struct LaneK {
     uint32_t mask;
-    uint32_t nsh;
+    uint32_t ksh, km;     // 32 - N and 3 << (N-4)
     uint32_t t0, t1, t2, t3;
+    uint32_t cm;
 };
-NQ_HD inline LaneK lane_consts(const Sched &S) { return LaneK{S.mask, S.nsh, S.t[0], S.t[1], S.t[2], S.t[3]}; }
-NQ_HD inline LaneK idle_lane() { return LaneK{0u, 0u, 0u, 0u, 0u, 0u}; }
+NQ_HD inline LaneK lane_k(uint32_t N, uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3)
+{
+    return LaneK{(1u << N) - 1u, 32u - N, 3u << (N - 4u), t0, t1, t2, t3, vconst(kColMask)};
+}
+NQ_HD inline LaneK lane_consts(const Sched &S) { return lane_k(S.nsh + 4u, S.t[0], S.t[1], S.t[2], S.t[3]); }
+NQ_HD inline LaneK idle_lane() { return LaneK{0u, 28u, 3u, 0u, 0u, 0u, 0u, vconst(kColMask)}; }
 
 // Parameter byte of the children of a frame, in bits 24-31 (bits 0-23 are
 // not defined), from the frame word (top bits: the frame's r).  On the
@@ -99,7 +139,7 @@ NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t W)
     const uint32_t sel = __builtin_amdgcn_ubfe(W, 4u, 27u);
     const uint32_t a = __builtin_amdgcn_perm(L.t1, L.t0, sel);
     const uint32_t b = __builtin_amdgcn_perm(L.t3, L.t2, sel);
-    return (int32_t)W < 0 ? b : a;
+    return bop3<kSel>((uint32_t)((int32_t)W >> 31), b, a);
 #else
     return table_byte(L.t0, L.t1, L.t2, L.t3, W >> kRShift) << kQShift;
 #endif
@@ -184,15 +224,16 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const uint32_t rest = W ^ bit;
     const uint32_t s = (C >> 24) & 3u;
     const uint32_t nld = ((w.ld | bit) << s) | ((C >> 26) & 3u);
-    const uint32_t nrd = ((w.rd | bit) >> s) | (((C >> 28) & 3u) << L.nsh);
+    const uint32_t nrd = bop3<kOrAnd>((w.rd | bit) >> s, C >> L.ksh, L.km);
     const uint32_t ncol = C | bit;
-    const uint32_t occ = nld | nrd | (C >> 31);          // JMN: column 0 of a jmark child
-    uint32_t nf = L.mask & ~(occ | ncol);
+    const uint32_t occ = bop3<kOr3>(nld, nrd, C >> 31);  // JMN: column 0 of a jmark child
+    uint32_t nf = bop3<kAndNotOr>(L.mask, occ, ncol);
 #if defined(__HIP_DEVICE_COMPILE__)
     asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
 #endif
     const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);   // all ones when the frame has no prune
-    const bool lok = ((L.mask & ~((nld << 1) | (nrd >> 1) | ncol)) | nla) != 0u;
+    const uint32_t look = bop3<kOr3>(nld << 1, nrd >> 1, ncol);
+    const bool lok = bop3<kAndNotOrC>(L.mask, look, nla) != 0u;
     const uint32_t hiW = W & ~kAvailMask;                // r << 28
     const uint32_t qc = child_q(L, W);
     const bool inner = W >= (1u << kRShift);             // children not on the terminal row
@@ -212,7 +253,7 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
     const uint32_t A = w.A;
     const uint32_t Wc = hiW + nf - (1u << kRShift);      // the child: r - 1, candidates nf
-    const uint32_t colc = (ncol & kColMask) | (qc & ~kColMask);
+    const uint32_t colc = bop3<kSel>(L.cm, ncol, qc);    // columns from ncol, parameter byte from qc
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
     w.col = down ? colc : (up ? top.z : C);

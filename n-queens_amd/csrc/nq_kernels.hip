@@ -183,7 +183,7 @@ constexpr uint32_t kCountFlush = 1u << 31;
 // Steps between two looks at the wave's lanes: a lane whose walk ends sits
 // on the inert frame and waits at most this many steps for its next item;
 // the steps in between carry no wave-wide check.
-constexpr int kStepsPerCheck = 16;
+constexpr int kStepsPerCheck = 32;
 
 // Each lane walks one item and holds the next one, loaded while it walks:
 // when its walk ends the lane switches to the held item without waiting for
@@ -232,7 +232,7 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
             if (T == 0u) {
                 if (p0.w & p1.z) atomicAdd(raw + c, 1ull);   // a root on the terminal row
             } else if ((p0.w & kAvailMask) != 0u) {
-                L = LaneK{(1u << N) - 1u, N - 4u, p2.x, p2.y, p2.z, p2.w};
+                L = lane_k(N, p2.x, p2.y, p2.z, p2.w);
                 w = Walk{p0.x, p0.y, p0.z, p0.w, st.empty()};
                 have = true;
             }
