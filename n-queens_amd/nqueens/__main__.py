@@ -1,8 +1,10 @@
-"""python -m nqueens [nmin] [nmax] [--mode plain|batched] [--scheduler fifo|mlp] [--preset P]
+"""python -m nqueens [nmin] [nmax] [--mode plain|batched] [--scheduler fifo|mlp] [--preset P] [--chunks K]
 
 Prints the reference solver's table (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:2115-2160)
 -- N, Total, Unique, hh:mm:ss.ms and ok/ng against A000170 (18Py:2089) --
-with Unique filled in rather than 0.  Every count runs on the GPU.
+with Unique filled in rather than 0.  Every count runs on the GPU.  With
+--chunks K each count runs as K device runs over interleaved slices of the
+constellations and prints a progress line after each.
 """
 from __future__ import annotations
 
@@ -29,6 +31,7 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
     ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="mlp")
     ap.add_argument("--preset", type=int, default=None)
+    ap.add_argument("--chunks", type=int, default=1)
     a = ap.parse_args(argv)
     nmax = a.nmax if a.nmax is not None else a.nmin
     ctx = solver.default_context()
@@ -36,7 +39,12 @@ def main(argv=None) -> int:
     bad = 0
     for N in range(a.nmin, nmax + 1):
         t0 = time.perf_counter()
-        total, unique = solver.count(N, preset=a.preset, mode=a.mode, scheduler=a.scheduler, ctx=ctx)
+        def progress(done, k, partial, N=N, t0=t0):
+            print(f"  N={N}: chunk {done}/{k} counted, partial total {partial}, "
+                  f"{time.perf_counter() - t0:.1f} s", flush=True)
+
+        total, unique = solver.count(N, preset=a.preset, mode=a.mode, scheduler=a.scheduler, ctx=ctx,
+                                     chunks=a.chunks, progress=progress if a.chunks > 1 else None)
         dt = time.perf_counter() - t0
         ok = "ok" if EXPECTED.get(N) == total else f"ng({total}!={EXPECTED.get(N)})"
         bad += ok != "ok"

@@ -4,6 +4,9 @@
 
 Loads the constellations of (N, preset) once and runs the chosen kernel
 `reps` times outside a graph, so that every dispatch is traced on its own.
+With --calib-mib M it first copies M MiB from one device buffer to another
+(torch, one plain copy kernel: M MiB read, M MiB written), the reference
+that the FETCH_SIZE / WRITE_SIZE counters of the other kernels are scaled by.
 """
 from __future__ import annotations
 
@@ -24,7 +27,16 @@ def main(argv=None) -> None:
     ap.add_argument("--split", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--lds", type=int, default=None)
+    ap.add_argument("--calib-mib", type=int, default=0)
     a = ap.parse_args(argv)
+    if a.calib_mib > 0:
+        import torch
+        src = torch.ones(a.calib_mib << 18, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.calib_mib << 20}), flush=True)
+        del src, dst
     ctx = default_context()
     rec = gen_constellations(a.n, a.preset)
     ctx.load(a.n, rec)

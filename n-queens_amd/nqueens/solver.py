@@ -104,20 +104,35 @@ def exec_solutions(N: int, constellations: list, use_gpu: bool = True) -> None:
 
 
 def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
-          sched: Sched | None = None, ctx: Context | None = None) -> tuple[int, int]:
-    """Total and unique N-Queens solutions for 6 <= N <= 23, counted on the GPU."""
+          sched: Sched | None = None, ctx: Context | None = None, chunks: int = 1,
+          progress=None) -> tuple[int, int]:
+    """Total and unique N-Queens solutions for 6 <= N <= 23, counted on the GPU.
+
+    chunks > 1 counts the constellations in that many interleaved slices, one
+    device run each (as the reference counts fixed-size chunks of its list,
+    359Py:5621-5626), and calls progress(done, chunks, partial_total) after
+    each: a long count then reports as it goes and no single launch holds the
+    GPU for the whole run."""
     _check_n(N)
+    if chunks < 1:
+        raise ValueError("chunks must be >= 1")
     ctx = ctx or default_context()
     rec = _cons.gen_constellations(N, preset or preset_for(N))
-    order = None
-    if scheduler == "mlp":
-        from .costmodel import default_model
-        order = default_model().order(N, rec, ctx=ctx)
-    elif scheduler != "fifo":
+    if scheduler not in ("fifo", "mlp"):
         raise ValueError(f"unknown scheduler {scheduler!r}")
     if mode == "batched" and sched is None:
         from .autotune import best_sched
         sched = best_sched(N, preset or preset_for(N))
-    _, _, total = count_tasks(N, rec, mode=mode, sched=sched, order=order, ctx=ctx)
+    total = 0
+    for i in range(chunks):
+        part = rec[i::chunks] if chunks > 1 else rec
+        order = None
+        if scheduler == "mlp" and part.shape[0]:
+            from .costmodel import default_model
+            order = default_model().order(N, part, ctx=ctx)
+        _, _, t = count_tasks(N, part, mode=mode, sched=sched, order=order, ctx=ctx)
+        total += t
+        if progress is not None:
+            progress(i + 1, chunks, total)
     f90, f180 = ctx.symmetric_counts(N)
     return total, unique_count(total, f90, f180)

@@ -517,7 +517,7 @@ static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
             lnext = nxt;
             if (!avail) continue;
         } else if (ptn == 4) {
-            return -2;   /* P5 targets (fids 8-11) never occur for N <= 21 */
+            return -2;   /* P5 targets (fids 8-11): none for N <= 23 at presets 5-6 */
         }
         if (use_blocks) {
             while (avail) {

@@ -213,3 +213,11 @@ def test_largest_n_oracle_sample(ctx, oracle_samples, N, mode):
 def test_largest_n_full_count(ctx, known, N):
     total, unique = nqueens.count(N, ctx=ctx)
     assert [total, unique] == list(known["readme_total_unique"][str(N)])
+
+
+def test_count_in_chunks(ctx, known):
+    """Interleaved slices counted one device run each add up to the whole."""
+    seen = []
+    total, unique = nqueens.count(16, ctx=ctx, chunks=3, progress=lambda d, k, t: seen.append((d, k, t)))
+    assert [total, unique] == known["readme_total_unique"]["16"]
+    assert [d for d, _, _ in seen] == [1, 2, 3] and seen[-1][2] == total
