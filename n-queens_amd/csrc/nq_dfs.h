@@ -232,7 +232,15 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
 #endif
     const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);   // all ones when the frame has no prune
-    const uint32_t look = bop3<kOr3>(nld << 1, nrd >> 1, ncol);
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t nld2;
+    // nld << 1 as an add: v_add_u32 issues at the full rate, v_lshlrev_b32
+    // at half (the compiler turns x + x back into a shift)
+    asm("v_add_u32 %0, %1, %1" : "=v"(nld2) : "v"(nld));
+#else
+    const uint32_t nld2 = nld << 1;
+#endif
+    const uint32_t look = bop3<kOr3>(nld2, nrd >> 1, ncol);
     const bool lok = bop3<kAndNotOrC>(L.mask, look, nla) != 0u;
     const uint32_t hiW = W & ~kAvailMask;                // r << 28
     const uint32_t qc = child_q(L, W);
