@@ -86,6 +86,23 @@ int nq_ctx_load(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const uint
  * 18Py:2116-2160). */
 int nq_ctx_load_mixed(nq_ctx *ctx, const int32_t *N, const uint32_t *rec4, uint64_t m, const uint32_t *order);
 
+/* Preset expansion on the device: each of the m records of preset p (the
+ * reference's 16-byte constellation records) becomes one record per free
+ * column of its start row, and this repeats `levels` times (1..8): the
+ * records of preset p + levels, in input order, columns ascending.  Replaces
+ * the recursion of set_pre_queens beyond preset p (18Py:1535-1613) for the
+ * records gen_constellations produced at preset p (18Py:1752-1781).
+ * With out4 == NULL only *m_out (the record count) is set; otherwise cap must
+ * be at least that count (NQ_ECAPACITY, with *m_out set, when it is not).
+ * A record whose start row is pre-placed or below the board, or a child field
+ * beyond 32 bits, is NQ_EUNSUPPORTED. */
+int nq_expand_records(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, int levels, uint32_t *out4,
+                      uint64_t cap, uint64_t *m_out);
+/* nq_ctx_load of the records nq_expand_records would produce, expanded and
+ * decoded on the device without a round trip through the host; *m_out gets
+ * the number of records loaded (may be NULL). */
+int nq_ctx_load_expanded(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, int levels, uint64_t *m_out);
+
 /* Per-record split depths for the batched kernel (used when
  * sched->split_depth == -1): the cost model bins constellations by predicted
  * work and splits the heavy ones deeper.  No reference counterpart (the

@@ -183,8 +183,46 @@ int nq_device_info(nq_ctx *c, int32_t *out4)
 
 namespace {
 
-// Shared body of nq_ctx_load (one N) and nq_ctx_load_mixed (N per record).
-int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_t m, const uint32_t *order)
+// Expands the m records at dev[0] by `levels` preset queens on the device
+// (two call-owned buffers, ping-pong); the result ends in dev[*cur], its
+// count in *m_io.
+int expand_device(nq_ctx *c, int N, nqk_buf::Buf<uint4> dev[2], int *cur, uint64_t *m_io, int levels)
+{
+    nqk_buf::Buf<uint32_t> cnt, off;
+    nqk_buf::Buf<unsigned long long> total;
+    NQ_HIPCHK(total.reserve(1));
+    uint64_t m = *m_io;
+    int a = 0;
+    for (int lv = 0; lv < levels && m; lv++) {
+        NQ_HIPCHK(cnt.reserve(m));
+        NQ_HIPCHK(off.reserve(m));
+        NQ_HIPCHK(nqk::launch_fill_u32(c->stream, c->err.p, 1, 0));
+        NQ_HIPCHK(nqk::launch_preset(c->stream, false, N, dev[a].p, (uint32_t)m, cnt.p, nullptr, nullptr, c->err.p));
+        NQ_HIPCHK(nqk::launch_scan(c->stream, cnt.p, (uint32_t)m, off.p, total.p));
+        unsigned long long n = 0;
+        uint32_t err = 0;
+        NQ_HIPCHK(hipMemcpyAsync(&n, total.p, 8, hipMemcpyDeviceToHost, c->stream));
+        NQ_HIPCHK(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+        NQ_HIPCHK(hipStreamSynchronize(c->stream));
+        if (err) return nq_fail(NQ_EUNSUPPORTED, "preset expansion failed at level %d (status mask 0x%x: "
+                                "4=start row pre-placed or below the board, 64=field beyond 32 bits)", lv + 1, err);
+        if (n >= (1ull << 31)) return nq_fail(NQ_ECAPACITY, "preset expansion gives %llu records (limit 2^31)", n);
+        NQ_HIPCHK(dev[a ^ 1].reserve(n ? n : 1));
+        NQ_HIPCHK(nqk::launch_preset(c->stream, true, N, dev[a].p, (uint32_t)m, cnt.p, off.p, dev[a ^ 1].p,
+                                     c->err.p));
+        NQ_HIPCHK(hipStreamSynchronize(c->stream));
+        a ^= 1;
+        m = n;
+    }
+    *cur = a;
+    *m_io = m;
+    return 0;
+}
+
+// Shared body of nq_ctx_load (one N), nq_ctx_load_mixed (N per record) and
+// nq_ctx_load_expanded (`levels` preset queens added on the device first).
+int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_t m, const uint32_t *order,
+              int levels = 0, uint64_t *m_out = nullptr)
 {
     if (int r = check_ctx(c)) return r;
     if (!Ns && (N < nqt::kMinN || N > nqt::kMaxN)) return nq_fail(NQ_EINVAL, "N=%d outside the u32 path (%d..%d)", N, nqt::kMinN, nqt::kMaxN);
@@ -200,6 +238,16 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
     }
     if (m && !rec4) return nq_fail(NQ_EINVAL, "null record pointer");
     if (m >= (1ull << 31)) return nq_fail(NQ_EINVAL, "too many records (%llu)", (unsigned long long)m);
+    // records expanded on the device (nq_ctx_load_expanded): the set to decode
+    nqk_buf::Buf<uint4> xdev[2];
+    int xcur = 0;
+    if (levels > 0) {
+        NQ_HIPCHK(c->err.reserve(1));
+        NQ_HIPCHK(xdev[0].reserve(m ? m : 1));
+        if (m) NQ_HIPCHK(hipMemcpyAsync(xdev[0].p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
+        if (int r = expand_device(c, N, xdev, &xcur, &m, levels)) return r;
+    }
+    if (m_out) *m_out = m;
     if (order) {
         std::vector<uint8_t> seen(m, 0);
         for (uint64_t a = 0; a < m; a++) {
@@ -231,7 +279,10 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
         order = ord.data();
     }
     if (m) {
-        NQ_HIPCHK(hipMemcpyAsync(c->rec.p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
+        if (levels > 0)
+            NQ_HIPCHK(hipMemcpyAsync(c->rec.p, xdev[xcur].p, m * 16, hipMemcpyDeviceToDevice, c->stream));
+        else
+            NQ_HIPCHK(hipMemcpyAsync(c->rec.p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
         NQ_HIPCHK(hipMemcpyAsync(c->order.p, order, m * 4, hipMemcpyHostToDevice, c->stream));
         if (Ns) NQ_HIPCHK(hipMemcpyAsync(c->nsz.p, n8.data(), m, hipMemcpyHostToDevice, c->stream));
     }
@@ -274,6 +325,37 @@ int nq_ctx_load_mixed(nq_ctx *c, const int32_t *Ns, const uint32_t *rec4, uint64
 {
     if (m && !Ns) return nq_fail(NQ_EINVAL, "null N array");
     return load_impl(c, 0, Ns, rec4, m, order);
+}
+
+int nq_ctx_load_expanded(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, int levels, uint64_t *m_out)
+{
+    if (levels < 1 || levels > 8) return nq_fail(NQ_EINVAL, "levels must be 1..8, got %d", levels);
+    return load_impl(c, N, nullptr, rec4, m, nullptr, levels, m_out);
+}
+
+int nq_expand_records(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, int levels, uint32_t *out4,
+                      uint64_t cap, uint64_t *m_out)
+{
+    if (int r = check_ctx(c)) return r;
+    if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside %d..%d", N, nqt::kMinN, nqt::kMaxN);
+    if (levels < 1 || levels > 8) return nq_fail(NQ_EINVAL, "levels must be 1..8, got %d", levels);
+    if (!m_out) return nq_fail(NQ_EINVAL, "null count pointer");
+    if (m && !rec4) return nq_fail(NQ_EINVAL, "null record pointer");
+    if (m >= (1ull << 31)) return nq_fail(NQ_EINVAL, "too many records (%llu)", (unsigned long long)m);
+    nqk_buf::Buf<uint4> dev[2];
+    int cur = 0;
+    NQ_HIPCHK(c->err.reserve(1));
+    NQ_HIPCHK(dev[0].reserve(m ? m : 1));
+    if (m) NQ_HIPCHK(hipMemcpyAsync(dev[0].p, rec4, m * 16, hipMemcpyHostToDevice, c->stream));
+    uint64_t n = m;
+    if (int r = expand_device(c, N, dev, &cur, &n, levels)) return r;
+    *m_out = n;
+    if (!out4) return 0;
+    if (cap < n) return nq_fail(NQ_ECAPACITY, "output holds %llu records, expansion gives %llu",
+                                (unsigned long long)cap, (unsigned long long)n);
+    if (n) NQ_HIPCHK(hipMemcpyAsync(out4, dev[cur].p, n * 16, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int nq_ctx_set_splits(nq_ctx *c, const uint8_t *depth, uint64_t m)

@@ -35,6 +35,8 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
                          unsigned long long *raw, unsigned long long *nodes_out);
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off,
                        unsigned long long *count);
+hipError_t launch_preset(hipStream_t s, bool write, int N, const uint4 *in, uint32_t m, uint32_t *cnt,
+                         const uint32_t *off, uint4 *out, uint32_t *err);
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,
                           const uint4 *items,
                           const unsigned long long *n_items, unsigned int *head, uint32_t chunk,

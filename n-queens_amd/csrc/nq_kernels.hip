@@ -143,6 +143,28 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     (void)raw;
 }
 
+// ---------------------------------------------------------------------------
+// preset expansion: each record of preset p becomes its records of preset
+// p + 1 (nqt::preset_children); pass 0 counts them per record, pass 1 writes
+// them at off[c].. in record order
+// ---------------------------------------------------------------------------
+template <bool WRITE>
+__global__ void k_preset(int N, const uint4 *__restrict__ in, uint32_t m, uint32_t *__restrict__ cnt,
+                         const uint32_t *__restrict__ off, uint4 *__restrict__ out, uint32_t *__restrict__ err)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    const uint4 r = in[c];
+    uint32_t n = 0;
+    uint4 *const o = WRITE ? out + off[c] : nullptr;
+    const int32_t st = nqt::preset_children(N, r.x, r.y, r.z, r.w, [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t sij) {
+        if (WRITE) o[n] = make_uint4(a, b, cc, sij);
+        n++;
+    });
+    if (st != nqt::kOk) atomicOr(err, 1u << st);
+    if (!WRITE) cnt[c] = n;
+}
+
 // exclusive scan of nitems[0..m) -> off, total -> *count (single block of 1024)
 __global__ void k_scan(const uint32_t *__restrict__ nitems, uint32_t m, uint32_t *__restrict__ off,
                        unsigned long long *__restrict__ count)
@@ -451,6 +473,16 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count)
 {
     k_scan<<<1, 1024, 0, s>>>(nitems, m, off, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_preset(hipStream_t s, bool write, int N, const uint4 *in, uint32_t m, uint32_t *cnt,
+                         const uint32_t *off, uint4 *out, uint32_t *err)
+{
+    if (m == 0) return hipSuccess;
+    const uint32_t blocks = (m + 255u) / 256u;
+    if (write) k_preset<true><<<blocks, 256, 0, s>>>(N, in, m, cnt, off, out, err);
+    else k_preset<false><<<blocks, 256, 0, s>>>(N, in, m, cnt, off, out, err);
     return hipGetLastError();
 }
 

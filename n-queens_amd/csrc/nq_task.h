@@ -254,6 +254,45 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     D.ld = ld; D.rd = rd; D.col = col; D.avail = fr;
 }
 
+// One more preset queen (set_pre_queens, 18Py:1535-1613, one level): a
+// record of preset p stands on its start row, which is not one of the
+// pre-placed rows k and l; every free column of that row gives a record of
+// preset p + 1, whose start row is the next row below that is not k or l
+// (the rows between move ld/rd by one each, 18Py:1591-1593).  The free set
+// is ~(ld | rd | col | LD >> (N-1-row) | RD << (N-1-row)) on the board
+// (18Py:1606-1611), LD and RD the j/l and j/k diagonals of the start
+// constellation (18Py:1749-1750).  emit(ld, rd, col, startijkl) receives the
+// children with columns ascending.  Returns kOk, kErrShift when the start row
+// is a pre-placed row or lies below the board, or kErrOverflow when a
+// child's field does not fit 32 bits (the record format is u32).
+constexpr int32_t kErrOverflow = 6;
+template <class Emit>
+NQ_HD inline int32_t preset_children(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, uint32_t r_sij,
+                                     Emit &&emit)
+{
+    if (N < kMinN || N > kMaxN) return kErrN;
+    const int row = (int)(r_sij >> 20);
+    const uint32_t ijkl = r_sij & 0xFFFFFu;
+    const int j = (int)((ijkl >> 10) & 31u), k = (int)((ijkl >> 5) & 31u), l = (int)(ijkl & 31u);
+    const int sh = N - 1 - row;
+    if (row == k || row == l || sh < 0 || j > N - 1 || l > N - 1 || k > N - 1) return kErrShift;
+    const uint64_t L = 1ull << (N - 1), mask = (1ull << N) - 1u;
+    const uint64_t LD = (L >> j) | (L >> l), RD = (L >> j) | (1ull << k);
+    uint64_t fr = ~((uint64_t)r_ld | r_rd | r_col | (LD >> sh) | (RD << sh)) & mask;
+    int32_t status = kOk;
+    while (fr) {
+        const uint64_t b = fr & (0ull - fr);
+        fr ^= b;
+        uint64_t nld = ((uint64_t)r_ld | b) << 1, nrd = ((uint64_t)r_rd | b) >> 1;
+        const uint64_t ncol = (uint64_t)r_col | b;
+        int nrow = row + 1;
+        while (nrow == k || nrow == l) { nld <<= 1; nrd >>= 1; nrow++; }
+        if ((nld | ncol) >> 32 || nrow > 31) { status = kErrOverflow; continue; }
+        emit((uint32_t)nld, (uint32_t)nrd, (uint32_t)ncol, ((uint32_t)nrow << 20) | ijkl);
+    }
+    return status;
+}
+
 // The per-record schedule the kernels read (2 x 16 bytes):
 //   words 0-3   byte r (r = 1..15) = parameter byte of depth T-r: the byte a
 //               frame with r = T-1-depth gives its children; byte 0 (the

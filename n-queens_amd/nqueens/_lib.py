@@ -61,6 +61,10 @@ SIGNATURES = {
     "nq_device_info": (ctypes.c_int, [_vp, _i32p]),
     "nq_ctx_load": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, _vp]),
     "nq_ctx_load_mixed": (ctypes.c_int, [_vp, _i32p, _u32p, ctypes.c_uint64, _vp]),
+    "nq_expand_records": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.c_int, _vp,
+                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "nq_ctx_load_expanded": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_uint64)]),
     "nq_ctx_set_splits": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"),
                                          ctypes.c_uint64]),
     "nq_ctx_run_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Timing)]),
@@ -166,6 +170,28 @@ class Context:
             ordp = _ptr(order)
         check(self.lib.nq_ctx_load(self.h, int(N), rec, rec.shape[0], ordp), "nq_ctx_load")
         self.N, self.m = int(N), rec.shape[0]
+
+    def expand_records(self, N: int, rec, levels: int = 1) -> np.ndarray:
+        """The records of preset p + levels from records of preset p,
+        expanded on the device (record order kept, columns ascending)."""
+        rec = as_records(rec)
+        n = ctypes.c_uint64(0)
+        check(self.lib.nq_expand_records(self.h, int(N), rec, rec.shape[0], int(levels), None, 0,
+                                         ctypes.byref(n)), "nq_expand_records")
+        out = np.zeros((int(n.value), 4), np.uint32)
+        check(self.lib.nq_expand_records(self.h, int(N), rec, rec.shape[0], int(levels), _ptr(out), out.shape[0],
+                                         ctypes.byref(n)), "nq_expand_records")
+        return out
+
+    def load_expanded(self, N: int, rec, levels: int) -> int:
+        """load() of the records expand_records() would give, expanded and
+        decoded on the device; returns the number of records loaded."""
+        rec = as_records(rec)
+        n = ctypes.c_uint64(0)
+        check(self.lib.nq_ctx_load_expanded(self.h, int(N), rec, rec.shape[0], int(levels), ctypes.byref(n)),
+              "nq_ctx_load_expanded")
+        self.N, self.m = int(N), int(n.value)
+        return self.m
 
     def load_mixed(self, Ns, rec, order=None) -> None:
         """A set mixing board sizes: Ns[i] is the N of record i."""

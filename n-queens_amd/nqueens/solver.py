@@ -24,7 +24,7 @@ from . import constellations as _cons
 from ._lib import Context, Sched, as_records
 
 __all__ = ["count", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
-           "preset_for", "MIN_N", "MAX_N"]
+           "preset_for", "constellations_for", "MIN_N", "MAX_N"]
 
 MIN_N, MAX_N = 6, 23
 
@@ -48,6 +48,16 @@ def preset_for(N: int) -> int:
     The walk keeps 16 parameter bytes per record, and a record of preset p
     has its terminal row at depth N-1-p, so N = 23 starts from preset 6."""
     return max(5, N - 17)
+
+
+def constellations_for(N: int, preset: int, ctx: Context | None = None):
+    """The constellation records of (N, preset): generated on the host up to
+    preset 5, the further preset queens placed on the device
+    (Context.expand_records, the same records as the host generator gives)."""
+    if preset <= 5:
+        return _cons.gen_constellations(N, preset)
+    ctx = ctx or default_context()
+    return ctx.expand_records(N, _cons.gen_constellations(N, 5), preset - 5)
 
 
 def default_sched(N: int | None = None) -> Sched:
@@ -117,7 +127,7 @@ def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler
     if chunks < 1:
         raise ValueError("chunks must be >= 1")
     ctx = ctx or default_context()
-    rec = _cons.gen_constellations(N, preset or preset_for(N))
+    rec = constellations_for(N, preset or preset_for(N), ctx)
     if scheduler not in ("fifo", "mlp"):
         raise ValueError(f"unknown scheduler {scheduler!r}")
     if mode == "batched" and sched is None:

@@ -96,3 +96,24 @@ extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weight
     }
     return 0;
 }
+
+// One level of preset expansion of every record (nqt::preset_children),
+// written to out (capacity cap records); returns the record count, or
+// -1 - status on a failed record, or -100 when out is too small.
+extern "C" long engine_expand(int N, const uint32_t *rec, long m, uint32_t *out, long cap)
+{
+    long n = 0;
+    for (long a = 0; a < m; a++) {
+        const uint32_t *r = rec + 4 * a;
+        bool full = false;
+        const int32_t st = nqt::preset_children(N, r[0], r[1], r[2], r[3],
+                                                [&](uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+                                                    if (n >= cap) { full = true; return; }
+                                                    uint32_t *o = out + 4 * n++;
+                                                    o[0] = x; o[1] = y; o[2] = z; o[3] = w;
+                                                });
+        if (st) return -1 - st;
+        if (full) return -100;
+    }
+    return n;
+}

@@ -38,6 +38,8 @@ def engine(tmp_path_factory):
     u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
     lib.engine_count.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u64p, u64p, ctypes.c_int, ctypes.c_int]
     lib.engine_count.restype = ctypes.c_int
+    lib.engine_expand.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u32p, ctypes.c_long]
+    lib.engine_expand.restype = ctypes.c_long
     return lib
 
 
@@ -87,3 +89,29 @@ def test_engine_n22_n23_sample(engine, N):
     rc, gw, gn = _run(engine, N, rec, split, priv)
     assert rc == 0
     assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+
+
+def _expand(engine, N, rec):
+    rec = np.ascontiguousarray(rec, dtype=np.uint32)
+    out = np.zeros((rec.shape[0] * N + 1, 4), np.uint32)
+    n = engine.engine_expand(N, rec, rec.shape[0], out, out.shape[0])
+    assert n >= 0, n
+    return out[:n]
+
+
+@pytest.mark.parametrize("N,p", [(N, 4) for N in (8, 11, 14)] + [(N, 5) for N in range(7, 18)] + [(12, 6), (15, 6)])
+def test_preset_expansion_matches_generator(engine, N, p):
+    # one level of the device expansion (nqt::preset_children) turns the
+    # preset-p records into exactly the preset-(p+1) record multiset
+    got = _expand(engine, N, C.gen_constellations(N, p))
+    want = C.gen_constellations(N, p + 1)
+    assert np.array_equal(C.records_sorted(got), C.records_sorted(want))
+
+
+def test_preset_expansion_reference_counts(engine, known):
+    # record counts of the reference's own logs: N=18 preset 6 (571,705) and
+    # preset 7 (3,653,940), N=21 preset 6 (2,025,282)
+    p6 = _expand(engine, 18, C.gen_constellations(18, 5))
+    assert p6.shape[0] == 571705
+    assert _expand(engine, 18, p6).shape[0] == 3653940
+    assert _expand(engine, 21, C.gen_constellations(21, 5)).shape[0] == 2025282

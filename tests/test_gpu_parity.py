@@ -221,3 +221,45 @@ def test_count_in_chunks(ctx, known):
     total, unique = nqueens.count(16, ctx=ctx, chunks=3, progress=lambda d, k, t: seen.append((d, k, t)))
     assert [total, unique] == known["readme_total_unique"]["16"]
     assert [d for d, _, _ in seen] == [1, 2, 3] and seen[-1][2] == total
+
+
+@pytest.mark.parametrize("N,p,levels", [(N, 5, 1) for N in (8, 12, 16, 19, 21, 23)] + [(14, 4, 1), (13, 5, 2), (16, 5, 3)])
+def test_device_preset_expansion(ctx, N, p, levels):
+    """nq_expand_records gives exactly the host generator's records of the
+    deeper preset (as multisets; the device keeps record order, columns
+    ascending)."""
+    got = ctx.expand_records(N, C.gen_constellations(N, p), levels)
+    want = C.gen_constellations(N, p + levels)
+    assert np.array_equal(C.records_sorted(got), C.records_sorted(want))
+
+
+def test_device_preset_expansion_reference_counts(ctx):
+    # record counts of the reference's logs: N=18 presets 6 and 7, N=21 preset 6
+    rec5 = C.gen_constellations(18, 5)
+    assert ctx.expand_records(18, rec5, 1).shape[0] == 571705
+    assert ctx.expand_records(18, rec5, 2).shape[0] == 3653940
+    assert ctx.expand_records(21, C.gen_constellations(21, 5), 1).shape[0] == 2025282
+
+
+@pytest.mark.parametrize("N,levels", [(12, 1), (15, 2), (17, 1)])
+def test_load_expanded_counts(ctx, oracle, known, N, levels):
+    """Records expanded and decoded on the device count the same as the
+    host-generated records of that preset, record by record."""
+    rec = C.gen_constellations(N, 5 + levels)
+    w, _, _ = oracle.count_records(N, rec)
+    m = ctx.load_expanded(N, C.gen_constellations(N, 5), levels)
+    assert m == rec.shape[0]
+    total, _ = ctx.run_batched(sched(2, 128, 1))
+    assert total == known["total_a000170"][str(N)] == int(w.sum())
+    gw, _ = ctx.results()
+    got = ctx.expand_records(N, C.gen_constellations(N, 5), levels)
+    key = lambda r, v: sorted(zip(map(tuple, r.tolist()), v.tolist()))
+    assert key(got, gw) == key(rec, w)
+
+
+def test_expansion_errors(ctx):
+    bad = np.array([[0, 0, 0, (3 << 20) | C.to_ijkl(5, 3, 3, 9)]], np.uint32)   # start row is row k
+    with pytest.raises(nqueens.NQError):
+        ctx.expand_records(12, bad, 1)
+    with pytest.raises(nqueens.NQError):
+        ctx.expand_records(12, C.gen_constellations(12, 5), 0)
