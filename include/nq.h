@@ -32,7 +32,7 @@ extern "C" {
 
 enum nq_status {
     NQ_OK = 0,
-    NQ_EINVAL = -1,        /* bad argument (N out of 6..23, NULL pointer, size) */
+    NQ_EINVAL = -1,        /* bad argument (N out of 6..24, NULL pointer, size) */
     NQ_EUNSUPPORTED = -2,  /* record outside the u32 path: P5 target (fids 8-11),
                               shift out of range, no terminal row */
     NQ_ECAPACITY = -3,     /* DFS stack / work-item buffer capacity exceeded */

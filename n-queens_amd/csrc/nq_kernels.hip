@@ -4,7 +4,7 @@
 //
 // Work unit: one DFS node is one extraction bit = avail & -avail of the
 // reference dfs_iter (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:368-562).
-// All masks are u32 (N <= 23); counts are u64.
+// All masks are u32 (N <= 24); counts are u64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "nq_task.h"

@@ -40,11 +40,11 @@ constexpr int kMinN = 6;
 // u32 masks: the board and col fit 24 bits (col's top byte carries the
 // walk's parameter byte); build_soa's rd bit (N-1-j) + (N-2-start) is set
 // only when j >= 2N-33-start, which keeps it at bit 30 or below
-constexpr int kMaxN = 23;
+constexpr int kMaxN = 24;
 
 enum : int32_t {
     kOk = 0,
-    kErrP5 = 1,          // target functionid 8..11 (not produced for N <= 23 at presets 5-6)
+    kErrP5 = 1,          // target functionid 8..11 (none for N <= 23 at presets 5-6, N = 24 at preset 7)
     kErrShift = 2,       // a shift of build_soa_for_range out of the u32 range
     kErrNoTerminal = 3,  // row sequence passes endmark without a base frame
     kErrDepth = 4,       // terminal depth beyond kMaxDepth

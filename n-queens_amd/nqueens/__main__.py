@@ -1,10 +1,12 @@
 """python -m nqueens [nmin] [nmax] [--mode plain|batched] [--scheduler fifo|mlp] [--preset P] [--chunks K]
+                  [--chunk-range FIRST LAST]
 
 Prints the reference solver's table (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:2115-2160)
 -- N, Total, Unique, hh:mm:ss.ms and ok/ng against A000170 (18Py:2089) --
 with Unique filled in rather than 0.  Every count runs on the GPU.  With
 --chunks K each count runs as K device runs over interleaved slices of the
-constellations and prints a progress line after each.
+constellations and prints a progress line after each; --chunk-range runs
+only some of them (a count split over several runs).
 """
 from __future__ import annotations
 
@@ -32,15 +34,31 @@ def main(argv=None) -> int:
     ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="mlp")
     ap.add_argument("--preset", type=int, default=None)
     ap.add_argument("--chunks", type=int, default=1)
+    ap.add_argument("--chunk-range", type=int, nargs=2, metavar=("FIRST", "LAST"), default=None,
+                    help="count only chunks FIRST..LAST-1 of --chunks and print their totals "
+                         "(the totals of all chunks, from any runs, add up to the N-Queens total)")
     a = ap.parse_args(argv)
+    if a.chunk_range is not None:
+        N = a.nmin
+        first, last = a.chunk_range
+        t0 = time.perf_counter()
+
+        def show(i, k, t):
+            print(f"N={N} chunk {i}/{k} total {t} ({time.perf_counter() - t0:.1f} s)", flush=True)
+
+        part = solver.count_chunks(N, a.chunks, first, last, preset=a.preset, mode=a.mode,
+                                   scheduler=a.scheduler, progress=show)
+        print(f"N={N} chunks {first}..{last - 1} of {a.chunks}: sum {sum(part)}", flush=True)
+        return 0
     nmax = a.nmax if a.nmax is not None else a.nmin
     ctx = solver.default_context()
     print(" N:             Total           Unique         hh:mm:ss.ms")
     bad = 0
     for N in range(a.nmin, nmax + 1):
         t0 = time.perf_counter()
-        def progress(done, k, partial, N=N, t0=t0):
-            print(f"  N={N}: chunk {done}/{k} counted, partial total {partial}, "
+        def progress(i, k, t, N=N, t0=t0, acc=[0]):
+            acc[0] += t
+            print(f"  N={N}: chunk {i + 1}/{k} counted, partial total {acc[0]}, "
                   f"{time.perf_counter() - t0:.1f} s", flush=True)
 
         total, unique = solver.count(N, preset=a.preset, mode=a.mode, scheduler=a.scheduler, ctx=ctx,

@@ -23,10 +23,10 @@ import numpy as np
 from . import constellations as _cons
 from ._lib import Context, Sched, as_records
 
-__all__ = ["count", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
+__all__ = ["count", "count_chunks", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
            "preset_for", "constellations_for", "MIN_N", "MAX_N"]
 
-MIN_N, MAX_N = 6, 23
+MIN_N, MAX_N = 6, 24
 
 _ctx: dict[int, Context] = {}
 _ctx_lock = threading.Lock()
@@ -46,7 +46,9 @@ def preset_for(N: int) -> int:
     """Preset-queen depth: 5, as 18Py always uses (:2139), up to N = 22.
 
     The walk keeps 16 parameter bytes per record, and a record of preset p
-    has its terminal row at depth N-1-p, so N = 23 starts from preset 6."""
+    has its terminal row at depth N-1-p, so N = 23 starts from preset 6 and
+    N = 24 from preset 7 (where, unlike at preset 5, no record needs the P5
+    functionids)."""
     return max(5, N - 17)
 
 
@@ -113,36 +115,69 @@ def exec_solutions(N: int, constellations: list, use_gpu: bool = True) -> None:
         c["solutions"] = int(v)
 
 
+def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *, preset: int | None = None,
+                 mode: str = "batched", scheduler: str = "fifo", sched: Sched | None = None,
+                 ctx: Context | None = None, progress=None) -> list[int]:
+    """Weighted totals of chunks first..last-1 of a count split in `chunks`.
+
+    Chunk i holds the constellations that descend from every chunks-th
+    record of preset min(preset, 5), starting at record i; deeper presets are
+    expanded on the device.  The chunks partition the constellation set, so
+    the totals of all chunks add up to the N-Queens total, whichever runs
+    (or processes) count them: a long count can be split over several runs
+    and resumed.  progress(i, chunks, total_i) is called after each chunk."""
+    _check_n(N)
+    last = chunks if last is None else last
+    if not (chunks >= 1 and 0 <= first <= last <= chunks):
+        raise ValueError(f"chunk range {first}..{last} of {chunks}")
+    if scheduler not in ("fifo", "mlp"):
+        raise ValueError(f"unknown scheduler {scheduler!r}")
+    ctx = ctx or default_context()
+    preset = preset or preset_for(N)
+    base = _cons.gen_constellations(N, min(preset, 5))
+    levels = preset - min(preset, 5)
+    if mode == "batched" and sched is None:
+        from .autotune import best_sched
+        sched = best_sched(N, preset)
+    totals = []
+    for i in range(first, last):
+        part = base[i::chunks] if chunks > 1 else base
+        if levels and scheduler == "fifo":
+            ctx.load_expanded(N, part, levels)
+            if mode == "plain":
+                t, _ = ctx.run_plain()
+            elif mode == "batched":
+                t, _ = ctx.run_batched(sched)
+            else:
+                raise ValueError(f"unknown mode {mode!r}")
+        else:
+            rec = ctx.expand_records(N, part, levels) if levels else part
+            order = None
+            if scheduler == "mlp" and rec.shape[0]:
+                from .costmodel import default_model
+                order = default_model().order(N, rec, ctx=ctx)
+            _, _, t = count_tasks(N, rec, mode=mode, sched=sched, order=order, ctx=ctx)
+        totals.append(int(t))
+        if progress is not None:
+            progress(i, chunks, int(t))
+    return totals
+
+
 def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
           sched: Sched | None = None, ctx: Context | None = None, chunks: int = 1,
           progress=None) -> tuple[int, int]:
-    """Total and unique N-Queens solutions for 6 <= N <= 23, counted on the GPU.
+    """Total and unique N-Queens solutions for 6 <= N <= 24, counted on the GPU.
 
-    chunks > 1 counts the constellations in that many interleaved slices, one
-    device run each (as the reference counts fixed-size chunks of its list,
-    359Py:5621-5626), and calls progress(done, chunks, partial_total) after
-    each: a long count then reports as it goes and no single launch holds the
-    GPU for the whole run."""
+    chunks > 1 counts the constellations in that many slices, one device run
+    each (as the reference counts fixed-size chunks of its list,
+    359Py:5621-5626; see count_chunks), and calls progress(i, chunks,
+    chunk_total) after each: a long count then reports as it goes and no
+    single launch holds the GPU for the whole run."""
     _check_n(N)
     if chunks < 1:
         raise ValueError("chunks must be >= 1")
     ctx = ctx or default_context()
-    rec = constellations_for(N, preset or preset_for(N), ctx)
-    if scheduler not in ("fifo", "mlp"):
-        raise ValueError(f"unknown scheduler {scheduler!r}")
-    if mode == "batched" and sched is None:
-        from .autotune import best_sched
-        sched = best_sched(N, preset or preset_for(N))
-    total = 0
-    for i in range(chunks):
-        part = rec[i::chunks] if chunks > 1 else rec
-        order = None
-        if scheduler == "mlp" and part.shape[0]:
-            from .costmodel import default_model
-            order = default_model().order(N, part, ctx=ctx)
-        _, _, t = count_tasks(N, part, mode=mode, sched=sched, order=order, ctx=ctx)
-        total += t
-        if progress is not None:
-            progress(i + 1, chunks, total)
+    total = sum(count_chunks(N, chunks, preset=preset, mode=mode, scheduler=scheduler, sched=sched, ctx=ctx,
+                             progress=progress))
     f90, f180 = ctx.symmetric_counts(N)
     return total, unique_count(total, f90, f180)

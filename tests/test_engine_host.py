@@ -76,6 +76,17 @@ def test_engine_largest_boards_sample(engine, N):
     assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
 
 
+def test_engine_n24_sample(engine):
+    # N=24 at preset 7 (terminal depth 16, 22-bit candidate sets, col's
+    # board bits reach bit 23 under the parameter byte)
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "oracle_sample_n24.npz") as z:
+        rec, w, nodes = z["rec24"][::28], z["w24"][::28], z["nodes24"][::28]
+    rc, gw, gn = _run(engine, 24, rec, 1, 0)
+    assert rc == 0
+    assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+
+
 @pytest.mark.parametrize("N", [22, 23])
 def test_engine_n22_n23_sample(engine, N):
     # N=22 (preset 5, terminal depth 16, 20-bit candidate sets) and N=23

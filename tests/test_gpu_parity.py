@@ -110,6 +110,15 @@ def test_symmetric_counts(ctx, oracle, N):
     assert ctx.symmetric_counts(N) == oracle.symmetric_counts(N)
 
 
+def test_symmetric_counts_n24(ctx, known):
+    # the oracle's N=24 values (43 s on one core, recomputed by
+    # tests/test_oracle.py::test_unique_table_through_burnside[24])
+    f90, f180 = ctx.symmetric_counts(24)
+    assert (f90, f180) == (3328, 11675080)
+    total, unique = known["readme_total_unique"]["24"]
+    assert total + f180 + 2 * f90 == 8 * unique
+
+
 @pytest.mark.parametrize("N", range(6, 19))
 @pytest.mark.parametrize("mode", ["plain", "batched"])
 def test_count_api_matches_published_table(known, N, mode):
@@ -128,7 +137,7 @@ def test_empty_set_and_errors(ctx):
     assert ctx.run_plain()[0] == 0
     assert ctx.run_batched(sched())[0] == 0
     with pytest.raises(nqueens.NQError):
-        ctx.load(24, C.gen_constellations(12, 5))
+        ctx.load(25, C.gen_constellations(12, 5))
     # a P5 target (functionid 8..11) is refused, never counted
     bad = np.array([[0, 0, 0, (2 << 20) | C.to_ijkl(5, 3, 7, 9)]], np.uint32)
     assert nqueens.decode_host(21, bad)[0, 9] == 1
@@ -182,20 +191,21 @@ def test_per_record_split_depths(ctx, oracle):
 def oracle_samples():
     from tests.conftest import GOLDEN
     out = {}
-    for name in ("oracle_sample_n20_n21.npz", "oracle_sample_n22_n23.npz"):
+    for name in ("oracle_sample_n20_n21.npz", "oracle_sample_n22_n23.npz", "oracle_sample_n24.npz"):
         with np.load(GOLDEN / name) as z:
             out.update({k: z[k] for k in z.files})
     return out
 
 
 # The largest boards: every 400th record of N=20/21 (preset 5), every
-# 2,000th of N=22 (preset 5) and every 20,000th of N=23 (preset 6), with
+# 2,000th of N=22 (preset 5), every 20,000th of N=23 (preset 6) and every
+# 600,000th of N=24 (preset 7), with
 # cached oracle counts.  Long ld histories fill all 32 bits of the
 # lookahead's occupancy word here, which N <= 19 rarely does; N=22/23 use
 # terminal depth 16 and 20/21-bit candidate sets.  The plain kernel (one lane
 # per record) is left out at N=22/23, where one record is up to 4e8 nodes.
 @pytest.mark.parametrize("N,mode", [(20, "plain"), (20, "batched"), (21, "plain"), (21, "batched"),
-                                    (22, "batched"), (23, "batched")])
+                                    (22, "batched"), (23, "batched"), (24, "batched")])
 def test_largest_n_oracle_sample(ctx, oracle_samples, N, mode):
     rec = oracle_samples[f"rec{N}"]
     ctx.load(N, rec)
@@ -218,9 +228,15 @@ def test_largest_n_full_count(ctx, known, N):
 def test_count_in_chunks(ctx, known):
     """Interleaved slices counted one device run each add up to the whole."""
     seen = []
-    total, unique = nqueens.count(16, ctx=ctx, chunks=3, progress=lambda d, k, t: seen.append((d, k, t)))
+    total, unique = nqueens.count(16, ctx=ctx, chunks=3, progress=lambda i, k, t: seen.append((i, k, t)))
     assert [total, unique] == known["readme_total_unique"]["16"]
-    assert [d for d, _, _ in seen] == [1, 2, 3] and seen[-1][2] == total
+    assert [i for i, _, _ in seen] == [0, 1, 2] and sum(t for _, _, t in seen) == total
+    # a count split over separate calls (chunk ranges) adds up to the same total,
+    # also with the deeper presets expanded on the device
+    from nqueens.solver import count_chunks
+    parts = count_chunks(16, 5, 0, 2, ctx=ctx) + count_chunks(16, 5, 2, 5, ctx=ctx)
+    assert sum(parts) == total
+    assert sum(count_chunks(16, 4, preset=7, ctx=ctx)) == total
 
 
 @pytest.mark.parametrize("N,p,levels", [(N, 5, 1) for N in (8, 12, 16, 19, 21, 23)] + [(14, 4, 1), (13, 5, 2), (16, 5, 3)])

@@ -75,7 +75,7 @@ def test_decode_matches_oracle_build_soa(oracle, N, preset):
 def test_decode_rejects_out_of_range():
     rec = C.gen_constellations(12, 5)
     with pytest.raises(nqueens.NQError):
-        nqueens.decode_host(24, rec)
+        nqueens.decode_host(25, rec)
     bad = rec.copy()
     bad[:, 3] = (31 << 20) | (bad[:, 3] & 0xFFFFF)   # start row beyond the board
     d = nqueens.decode_host(12, bad)
@@ -110,7 +110,7 @@ def test_solver_argument_checks():
     with pytest.raises(ValueError):
         nqueens.count(5)
     with pytest.raises(ValueError):
-        nqueens.count(24)
+        nqueens.count(25)
     with pytest.raises(ValueError):
         nqueens.exec_solutions(8, [{"ld": 0, "rd": 0, "col": 0, "startijkl": 0}], use_gpu=False)
     with pytest.raises(ArithmeticError):

@@ -63,7 +63,7 @@ def test_bruteforce_pins_symmetric_counts(oracle, known, N):
         assert [total, unique] == known["readme_total_unique"][str(N)]
 
 
-@pytest.mark.parametrize("N", range(4, 24))
+@pytest.mark.parametrize("N", range(4, 25))
 def test_unique_table_through_burnside(oracle, known, N):
     total, unique = known["readme_total_unique"][str(N)]
     f90, f180 = oracle.symmetric_counts(N)
