@@ -28,6 +28,7 @@ def main(argv=None) -> None:
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--lds", type=int, default=None)
     ap.add_argument("--calib-mib", type=int, default=0)
+    ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="fifo")
     a = ap.parse_args(argv)
     if a.calib_mib > 0:
         import torch
@@ -39,7 +40,11 @@ def main(argv=None) -> None:
         del src, dst
     ctx = default_context()
     rec = gen_constellations(a.n, a.preset)
-    ctx.load(a.n, rec)
+    order = None
+    if a.scheduler == "mlp":
+        from .costmodel import default_model
+        order = default_model().order(a.n, rec, ctx=ctx)
+    ctx.load(a.n, rec, order)
     s = autotune.best_sched(a.n, a.preset)
     for k, v in (("split_depth", a.split), ("block", a.block), ("lds_stack", a.lds)):
         if v is not None:
