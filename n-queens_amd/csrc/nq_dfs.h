@@ -55,12 +55,12 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // frames
 // ---------------------------------------------------------------------------
 // A frame is {ld, rd, col, W}.  The frame word W holds the candidates
-// (avail, bits 0-27) and r = T-1-depth (bits 28-31): the frame's children are
-// on the terminal row exactly when r = 0, i.e. when W is below 2^28, and a
-// child's word is W's top bits less 2^28 plus its candidates.  col holds the occupied
-// columns in bits 0-23 (N <= 24) and the parameter byte of the frame's depth
-// in bits 24-31; the board mask drops that byte wherever col meets a board
-// set.
+// (avail, bits 0-27) and r = T-1-depth (bits 28-31): the frame's children
+// are on the terminal row exactly when r = 0, i.e. when W is below 2^28, and
+// a child's word is W's top bits less 2^28 plus its candidates.  col holds
+// the occupied columns in bits 0-23 (N <= 24) and the parameter byte of the
+// frame's depth in bits 24-31; the board mask drops that byte wherever col
+// meets a board set.
 constexpr uint32_t kAvailMask = 0x0FFFFFFFu;
 constexpr uint32_t kQShift = 24u;
 constexpr uint32_t kRShift = 28u;
