@@ -116,3 +116,13 @@ def test_solver_argument_checks():
     with pytest.raises(ArithmeticError):
         nqueens.unique_count(10, 0, 1)
     assert nqueens.unique_count(14200, 8, 80) == 1787
+    # chunk ranges are checked before any device work
+    from nqueens.solver import count_chunks, preset_for
+    for args in ((16, 0), (16, 4, 3, 2), (16, 4, 0, 5), (16, 4, -1, 2), (25, 4)):
+        with pytest.raises(ValueError):
+            count_chunks(*args)
+    with pytest.raises(ValueError):
+        nqueens.count(16, chunks=0)
+    # terminal depth N-1-preset stays within the walk's 16 parameter bytes
+    assert [preset_for(n) for n in (19, 22, 23, 24)] == [5, 5, 6, 7]
+    assert all(n - 1 - preset_for(n) <= 16 for n in range(6, 25))
