@@ -87,8 +87,9 @@ int size_items(nq_ctx *c, int split, int block)
     unsigned long long cnt = 0;
     NQ_HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     NQ_HIPCHK(hipStreamSynchronize(c->stream));
-    // item indices, the queue head and its claims past the end are u32
-    if (cnt >= (1ull << 31)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^31)", split, cnt);
+    // item indices and offsets are u32 (the queue head's claims past the end
+    // are checked against the launch in nq_ctx_run_batched)
+    if (cnt >= (1ull << 32)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^32)", split, cnt);
     NQ_HIPCHK(c->items.reserve(nqk::kItemU4 * cnt));
     c->item_total = cnt;
     c->sized_split = split;
@@ -443,6 +444,10 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         per_cu = std::min(per_cu, want);
     }
     const int grid = c->cu * per_cu;
+    // the u32 queue head ends at most one chunk per wave past the item count
+    if (c->item_total + (uint64_t)grid * (s.block / 64) * (uint64_t)s.chunk >= (1ull << 32))
+        return nq_fail(NQ_ECAPACITY, "%llu work items and %d claims of %d past the end overflow the u32 queue head",
+                       (unsigned long long)c->item_total, grid * (s.block / 64), s.chunk);
     if (s.use_graph) {
         nq_graph_key k;
         k.split = s.split_depth; k.block = s.block; k.lds = s.lds_stack; k.chunk = s.chunk; k.grid = grid;
