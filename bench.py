@@ -233,7 +233,9 @@ def big_count(nqueens, ctx, known, N: int, preset: int, scheduler: str) -> dict:
 
 def cpu_baseline(N: int, rec: np.ndarray, nodes: np.ndarray, seconds: float) -> dict:
     """Oracle dfs_iter (C, OpenMP dynamic,64) on every k-th record, k chosen
-    from the GPU node counts so that the sample is about `seconds` of work."""
+    from the GPU node counts so that the sample is about `seconds` of work;
+    the same sample is timed again with schedule(static), the Codon @par
+    default (18Py:1050)."""
     from oracle import oracle
     cores = len(os.sched_getaffinity(0))
     # calibrate on a small stride sample
@@ -250,10 +252,15 @@ def cpu_baseline(N: int, rec: np.ndarray, nodes: np.ndarray, seconds: float) -> 
     dt = time.perf_counter() - t0
     if not np.array_equal(sn, nodes[::stride]):
         raise SystemExit("cpu baseline: node counts disagree with the GPU")
+    # the same sample with the static schedule of the reference's Codon @par
+    t0 = time.perf_counter()
+    oracle.count_records(N, sample, threads=cores, schedule=0, nodes=False)
+    dts = time.perf_counter() - t0
     return {"value": round(float(sn.sum()) / dt, 1), "unit": "nodes/s", "cores": cores, "kind": "port",
             "sample": f"every {stride}th of the {rec.shape[0]} N={N} constellations "
                       f"({sample.shape[0]} records, {int(sn.sum())} nodes, {dt:.1f} s), OpenMP dynamic,64",
-            "seconds": round(dt, 2)}
+            "seconds": round(dt, 2),
+            "static_schedule": {"value": round(float(sn.sum()) / dts, 1), "seconds": round(dts, 2)}}
 
 
 if __name__ == "__main__":
