@@ -117,8 +117,9 @@ def exec_solutions(N: int, constellations: list, use_gpu: bool = True) -> None:
 
 def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *, preset: int | None = None,
                  mode: str = "batched", scheduler: str = "fifo", sched: Sched | None = None,
-                 ctx: Context | None = None, progress=None) -> list[int]:
-    """Weighted totals of chunks first..last-1 of a count split in `chunks`.
+                 ctx: Context | None = None, progress=None, ids=None) -> list[int]:
+    """Weighted totals of chunks first..last-1 (or of the chunk indices `ids`)
+    of a count split in `chunks`.
 
     Chunk i holds the constellations that descend from every chunks-th
     record of preset min(preset, 5), starting at record i; deeper presets are
@@ -130,6 +131,9 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
     last = chunks if last is None else last
     if not (chunks >= 1 and 0 <= first <= last <= chunks):
         raise ValueError(f"chunk range {first}..{last} of {chunks}")
+    ids = list(range(first, last)) if ids is None else [int(i) for i in ids]
+    if any(not 0 <= i < chunks for i in ids):
+        raise ValueError(f"chunk index outside 0..{chunks - 1}")
     if scheduler not in ("fifo", "mlp"):
         raise ValueError(f"unknown scheduler {scheduler!r}")
     ctx = ctx or default_context()
@@ -140,7 +144,7 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
         from .autotune import best_sched
         sched = best_sched(N, preset)
     totals = []
-    for i in range(first, last):
+    for i in ids:
         part = base[i::chunks] if chunks > 1 else base
         if levels and scheduler == "fifo":
             ctx.load_expanded(N, part, levels)

@@ -279,3 +279,18 @@ def test_expansion_errors(ctx):
         ctx.expand_records(12, bad, 1)
     with pytest.raises(nqueens.NQError):
         ctx.expand_records(12, C.gen_constellations(12, 5), 0)
+
+
+def test_resumed_count_from_progress_file(tmp_path, known, capsys):
+    """A count interrupted after two of four chunks resumes from its progress
+    log and ends with the published total and unique count."""
+    from nqueens.__main__ import main
+    from nqueens.progress import ChunkLog
+    log = tmp_path / "n14.tsv"
+    assert main(["14", "--chunks", "4", "--chunk-range", "0", "2", "--progress-file", str(log)]) == 0
+    assert ChunkLog(log, 14, 4, 5).missing() == [2, 3]
+    assert main(["14", "--chunks", "4", "--progress-file", str(log)]) == 0
+    out = capsys.readouterr().out
+    total, unique = known["readme_total_unique"]["14"]
+    assert f"{total:18d}{unique:17d}    ok" in out
+    assert ChunkLog(log, 14, 4, 5).total() == total
