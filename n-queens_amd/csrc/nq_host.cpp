@@ -96,10 +96,10 @@ int size_items(nq_ctx *c, int split, int block)
     return 0;
 }
 
-// The batched launch sequence: clear, expand (write), persistent DFS,
-// finalize.  The item offsets and count come from size_items, which runs
-// whenever the resident set, its split depths or the split change.  Events
-// 0/3 bracket the sequence, 1/2 the DFS kernel.
+// The batched launch sequence: clear, expand (count, scan, write), persistent
+// DFS, finalize: every run recomputes the whole expansion from the resident
+// records (size_items has only sized the item buffer).  Events 0/3 bracket
+// the sequence, 1/2 the DFS kernel.
 int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool events)
 {
     const uint32_t m = (uint32_t)c->m, sd = expand_slots(c);
@@ -112,6 +112,9 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
     if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, m, 0));
     const uint8_t *sp8 = s.split_depth < 0 ? c->splits.p : nullptr;
     const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
+    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
+                              su, c->nitems.p, nullptr, nullptr, nullptr, nullptr));
+    NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
     NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
                               su, c->nitems.p, c->off.p, c->items.p,
                               c->raw.p, c->nodes.p));
