@@ -87,30 +87,41 @@ def main() -> int:
         if v is not None:
             setattr(sched, k, v)
     sched.use_graph = args.graph
-    order = None
     mlp_info = None
+    ctx.load(N, rec)
+    params = None
     if args.scheduler == "mlp":
+        # the cost model orders the records on the device inside every timed
+        # count (features, forward pass, radix sort), before the batched run
         from nqueens import costmodel
         model = costmodel.default_model()
+        params = model.params
+        ctx.order_by_cost(params)
         t0 = time.perf_counter()
-        order = model.order(N, rec, ctx=ctx)
-        mlp_info = {"order_s": round(time.perf_counter() - t0, 4), "trained_on": model.trained_on}
-    ctx.load(N, rec, order)
+        for _ in range(5):
+            ctx.order_by_cost(params)
+        mlp_info = {"order_ms": round((time.perf_counter() - t0) / 5 * 1000, 3), "trained_on": model.trained_on,
+                    "in_timed_step": True}
+
+    def count_once(nodes_on: bool = False):
+        if params is not None:
+            ctx.order_by_cost(params)
+        return ctx.run_batched(sched, count_nodes=nodes_on)
 
     # node count of the workload (untimed, same kernel with the counter on)
-    total, _ = ctx.run_batched(sched, count_nodes=True)
+    total, _ = count_once(True)
     _, nodes = ctx.results(weighted=False, nodes=True)
     nodes_total = int(nodes.sum())
     assert total == expect_total, (total, expect_total)
     log(f"[bench] nodes={nodes_total} sched={dict((f, getattr(sched, f)) for f, _ in sched._fields_)}")
 
     for _ in range(args.warmup):
-        tot, _ = ctx.run_batched(sched)
+        tot, _ = count_once()
         assert tot == expect_total
     kernel_ms, run_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        tot, t = ctx.run_batched(sched)
+        tot, t = count_once()
         if tot != expect_total:
             raise SystemExit(f"wrong total {tot} != {expect_total}")
         kernel_ms.append(t.kernel_ms)
@@ -143,7 +154,9 @@ def main() -> int:
     if sched.use_graph:
         eager = nqueens.Sched(**{f: getattr(sched, f) for f, _ in sched._fields_})
         eager.use_graph = 0
-        ctx.load(N, rec, order)
+        ctx.load(N, rec)
+        if params is not None:
+            ctx.order_by_cost(params)
         kernel_ms = []
         for _ in range(max(2, args.steps)):
             tot, t = ctx.run_batched(eager)
@@ -210,16 +223,19 @@ def big_count(nqueens, ctx, known, N: int, preset: int, scheduler: str) -> dict:
     node rate; nodes from a node-counting run of the same set)."""
     from nqueens import autotune
     rec = nqueens.gen_constellations(N, preset)
-    order = None
+    ctx.load(N, rec)
+    params = None
     if scheduler == "mlp":
         from nqueens import costmodel
-        order = costmodel.default_model().order(N, rec, ctx=ctx)
-    ctx.load(N, rec, order)
+        params = costmodel.default_model().params
+        ctx.order_by_cost(params)
     s = autotune.best_sched(N, preset)
     s.use_graph = 1
     tot, _ = ctx.run_batched(s, count_nodes=True)
     _, nodes = ctx.results(weighted=False, nodes=True)
     t0 = time.perf_counter()
+    if params is not None:
+        ctx.order_by_cost(params)
     tot2, t = ctx.run_batched(s)
     wall = time.perf_counter() - t0
     want = known["readme_total_unique"][str(N)][0]

@@ -147,6 +147,11 @@ int nq_symmetric_counts(nq_ctx *ctx, int N, uint64_t *f90_out, uint64_t *f180_ou
 
 /* Features of the resident task set (row-major m x 24 floats, record order). */
 int nq_ctx_features(nq_ctx *ctx, float *feat_out);
+/* Processing order of the resident task set from the model, on the device:
+ * features, forward pass and a stable radix sort, predicted heaviest first
+ * (ties in record order), written to the order the batched kernel reads
+ * (and to order_out[m] unless NULL).  The order only schedules work. */
+int nq_ctx_order_by_cost(nq_ctx *ctx, const float *params, uint32_t *order_out);
 /* Forward pass: x[m x 24] -> y[m]. */
 int nq_mlp_forward(nq_ctx *ctx, const float *params, const float *x, uint64_t m, float *y_out);
 /* Mean-squared-error loss and its gradient w.r.t. the packed parameters. */

@@ -75,6 +75,10 @@ struct nq_ctx {
     bool has_splits = false;
     uint32_t split_min = 0;       // smallest per-record split depth
     nqk_buf::Buf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
+    // cost-model ordering on the device (nq_ctx_order_by_cost)
+    nqk_buf::Buf<float> cm_params, cm_x, cm_y, cm_ys;
+    nqk_buf::Buf<uint32_t> cm_idx;
+    nqk_buf::Buf<uint8_t> cm_tmp;
     int sized_split = -2;         // split depth the item buffer is sized for (-1 per-record, -2 none)
     uint64_t item_total = 0;
     // captured batched sequence

@@ -6,6 +6,7 @@
 // Parameter layout (NQ_MLP_NPARAM floats):
 //   W1[64][24] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include <stdint.h>
 #include <math.h>
 #include <algorithm>
@@ -220,6 +221,12 @@ hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x
     return hipGetLastError();
 }
 
+__global__ void k_iota(uint32_t *__restrict__ p, uint32_t m)
+{
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < m) p[a] = a;
+}
+
 }  // namespace nqk
 
 // ---------------------------------------------------------------------------
@@ -252,6 +259,36 @@ int nq_ctx_features(nq_ctx *c, float *feat_out)
     NQ_HIPCHK(hipMemcpyAsync(feat_out, f.p, m * nqk::kIn * sizeof(float), hipMemcpyDeviceToHost,
                              c->stream));
     NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int nq_ctx_order_by_cost(nq_ctx *c, const float *params, uint32_t *order_out)
+{
+    NQ_CTX_CHECK(c);
+    if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
+    if (!params) return nq_fail(NQ_EINVAL, "null parameters");
+    const uint32_t m = (uint32_t)c->m;
+    if (m == 0) return 0;
+    hipStream_t s = c->stream;
+    NQ_HIPCHK(c->cm_params.reserve(nqk::kNP));
+    NQ_HIPCHK(c->cm_x.reserve((size_t)m * nqk::kIn));
+    NQ_HIPCHK(c->cm_y.reserve(m));
+    NQ_HIPCHK(c->cm_ys.reserve(m));
+    NQ_HIPCHK(c->cm_idx.reserve(m));
+    NQ_HIPCHK(hipMemcpyAsync(c->cm_params.p, params, nqk::kNP * sizeof(float), hipMemcpyHostToDevice, s));
+    NQ_HIPCHK(nqk::launch_features(s, c->rec.p, c->root.p, c->sched.p, m, c->cm_x.p));
+    NQ_HIPCHK(nqk::launch_mlp_forward(s, c->cm_params.p, c->cm_x.p, m, c->cm_y.p, nullptr, nullptr));
+    nqk::k_iota<<<(m + 255) / 256, 256, 0, s>>>(c->cm_idx.p, m);
+    NQ_HIPCHK(hipGetLastError());
+    // predicted heaviest first; ties keep record order (the radix sort is stable)
+    size_t tmp = 0;
+    NQ_HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, c->cm_y.p, c->cm_ys.p, c->cm_idx.p,
+                                                           c->order.p, (int)m, 0, 32, s));
+    NQ_HIPCHK(c->cm_tmp.reserve(tmp ? tmp : 1));
+    NQ_HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(c->cm_tmp.p, tmp, c->cm_y.p, c->cm_ys.p, c->cm_idx.p,
+                                                           c->order.p, (int)m, 0, 32, s));
+    if (order_out) NQ_HIPCHK(hipMemcpyAsync(order_out, c->order.p, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+    NQ_HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
 

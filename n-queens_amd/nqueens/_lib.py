@@ -77,6 +77,7 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_uint64)]),
     "nq_ctx_features": (ctypes.c_int, [_vp, _f32p]),
     "nq_mlp_forward": (ctypes.c_int, [_vp, _f32p, _f32p, ctypes.c_uint64, _f32p]),
+    "nq_ctx_order_by_cost": (ctypes.c_int, [_vp, _f32p, _vp]),
     "nq_mlp_loss_grad": (ctypes.c_int, [_vp, _f32p, _f32p, _f32p, ctypes.c_uint64, _f32p, _f32p]),
     "nq_mlp_train": (ctypes.c_int, [_vp, _f32p, _f32p, _f32p, ctypes.c_uint64, _u32p, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_float, _f32p]),
@@ -242,6 +243,16 @@ class Context:
     def features(self) -> np.ndarray:
         out = np.zeros((self.m, NQ_MLP_IN), np.float32)
         check(self.lib.nq_ctx_features(self.h, out.reshape(-1)), "nq_ctx_features")
+        return out
+
+    def order_by_cost(self, params: np.ndarray, want: bool = False):
+        """Orders the resident set by the model's predicted cost on the device
+        (heaviest first); returns the order when `want`."""
+        params = np.ascontiguousarray(params, np.float32)
+        if params.shape != (NQ_MLP_NPARAM,):
+            raise ValueError(f"expected {NQ_MLP_NPARAM} parameters")
+        out = np.zeros(self.m, np.uint32) if want else None
+        check(self.lib.nq_ctx_order_by_cost(self.h, params, _ptr(out)), "nq_ctx_order_by_cost")
         return out
 
     def mlp_forward(self, params: np.ndarray, x: np.ndarray) -> np.ndarray:

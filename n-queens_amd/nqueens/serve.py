@@ -102,13 +102,10 @@ class RequestLoop:
         recs = [self.records(r.N) for r in batch]
         rec = np.concatenate(recs)
         Ns = np.concatenate([np.full(x.shape[0], r.N, np.int32) for x, r in zip(recs, batch)])
-        order = None
+        self.ctx.load_mixed(Ns, rec)
         if self.scheduler == "mlp":
             from .costmodel import default_model
-            self.ctx.load_mixed(Ns, rec)
-            pred = default_model().predict_features(self.ctx, self.ctx.features())
-            order = np.argsort(-pred, kind="stable").astype(np.uint32)
-        self.ctx.load_mixed(Ns, rec, order)
+            self.ctx.order_by_cost(default_model().params)   # on the device, heaviest first
         self.ctx.run_batched(self._sched([r.N for r in batch]), count_nodes=count_nodes)
         w, nodes = self.ctx.results(weighted=True, nodes=count_nodes)
         bounds = np.cumsum([0] + [x.shape[0] for x in recs])

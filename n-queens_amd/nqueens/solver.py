@@ -146,21 +146,19 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
     totals = []
     for i in ids:
         part = base[i::chunks] if chunks > 1 else base
-        if levels and scheduler == "fifo":
+        if levels:
             ctx.load_expanded(N, part, levels)
-            if mode == "plain":
-                t, _ = ctx.run_plain()
-            elif mode == "batched":
-                t, _ = ctx.run_batched(sched)
-            else:
-                raise ValueError(f"unknown mode {mode!r}")
         else:
-            rec = ctx.expand_records(N, part, levels) if levels else part
-            order = None
-            if scheduler == "mlp" and rec.shape[0]:
-                from .costmodel import default_model
-                order = default_model().order(N, rec, ctx=ctx)
-            _, _, t = count_tasks(N, rec, mode=mode, sched=sched, order=order, ctx=ctx)
+            ctx.load(N, part)
+        if scheduler == "mlp" and ctx.m:
+            from .costmodel import default_model
+            ctx.order_by_cost(default_model().params)       # on the device, heaviest first
+        if mode == "plain":
+            t, _ = ctx.run_plain()
+        elif mode == "batched":
+            t, _ = ctx.run_batched(sched)
+        else:
+            raise ValueError(f"unknown mode {mode!r}")
         totals.append(int(t))
         if progress is not None:
             progress(i, chunks, int(t))

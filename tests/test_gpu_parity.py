@@ -294,3 +294,18 @@ def test_resumed_count_from_progress_file(tmp_path, known, capsys):
     total, unique = known["readme_total_unique"]["14"]
     assert f"{total:18d}{unique:17d}    ok" in out
     assert ChunkLog(log, 14, 4, 5).total() == total
+
+
+@pytest.mark.parametrize("N", [13, 16])
+def test_device_cost_order(ctx, known, N):
+    """The device ordering (features, MLP forward, stable radix sort) is the
+    host's argsort of the same predictions, and leaves the count unchanged."""
+    from nqueens import costmodel
+    params = costmodel.default_model().params
+    ctx.load(N, C.gen_constellations(N, 5))
+    pred = ctx.mlp_forward(params, ctx.features())
+    want = np.argsort(-pred, kind="stable").astype(np.uint32)
+    got = ctx.order_by_cost(params, want=True)
+    assert np.array_equal(got, want)
+    total, _ = ctx.run_batched(sched(3, 128, 1, graph=1))
+    assert total == known["total_a000170"][str(N)]
