@@ -40,16 +40,18 @@ def main(argv=None) -> None:
         del src, dst
     ctx = default_context()
     rec = gen_constellations(a.n, a.preset)
-    order = None
+    ctx.load(a.n, rec)
+    params = None
     if a.scheduler == "mlp":
         from .costmodel import default_model
-        order = default_model().order(a.n, rec, ctx=ctx)
-    ctx.load(a.n, rec, order)
+        params = default_model().params
     s = autotune.best_sched(a.n, a.preset)
     for k, v in (("split_depth", a.split), ("block", a.block), ("lds_stack", a.lds)):
         if v is not None:
             setattr(s, k, v)
     for r in range(a.reps):
+        if params is not None:
+            ctx.order_by_cost(params)        # the cost-model order, made on the device as in bench.py
         if a.mode == "batched":
             tot, t = ctx.run_batched(s)
         else:
