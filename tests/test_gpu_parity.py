@@ -309,3 +309,24 @@ def test_device_cost_order(ctx, known, N):
     assert np.array_equal(got, want)
     total, _ = ctx.run_batched(sched(3, 128, 1, graph=1))
     assert total == known["total_a000170"][str(N)]
+
+
+def test_trained_model_gains_over_fifo(ctx, oracle):
+    """What the trained network (the shipped model, trained on N=16..19) gains
+    over FIFO: N=17's constellations as whole work items on 16,384 lanes,
+    list-scheduled with their measured node counts as costs.  FIFO ends 68 %
+    above the work bound, heaviest-first by the true counts 14 %."""
+    from nqueens import costmodel
+    rec = C.gen_constellations(17, 5)
+    _, _, nodes = oracle.count_records(17, rec)
+    cost = nodes.astype(np.float64)
+    ctx.load(17, rec)
+    order = ctx.order_by_cost(costmodel.default_model().params, want=True)
+    lanes = 16384
+    lower = cost.sum() / lanes
+    fifo = costmodel.simulated_makespan(cost, np.arange(cost.size), lanes) / lower
+    model = costmodel.simulated_makespan(cost, order, lanes) / lower
+    best = costmodel.simulated_makespan(cost, np.argsort(-cost, kind="stable"), lanes) / lower
+    print(f"makespan / work bound: FIFO {fifo:.3f}, model order {model:.3f}, true order {best:.3f}")
+    assert best <= model < fifo
+    assert model < 1.25 and fifo > 1.6
