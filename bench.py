@@ -64,8 +64,9 @@ def main() -> int:
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the count shards by constellation; see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (a count over "
+                                   "several GPUs would give each a share of the constellations and add "
+                                   "the totals)"}))
         return 2
 
     import nqueens

@@ -126,3 +126,17 @@ def test_solver_argument_checks():
     # terminal depth N-1-preset stays within the walk's 16 parameter bytes
     assert [preset_for(n) for n in (19, 22, 23, 24)] == [5, 5, 6, 7]
     assert all(n - 1 - preset_for(n) <= 16 for n in range(6, 25))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and a
+    JSON error line, before touching a device."""
+    import json
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parent.parent
+    res = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 2
+    msg = json.loads(res.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
