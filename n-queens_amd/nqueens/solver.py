@@ -43,13 +43,21 @@ def default_context(device: int = 0) -> Context:
 
 
 def preset_for(N: int) -> int:
-    """Preset-queen depth: 5, as 18Py always uses (:2139), up to N = 22.
+    """Preset-queen depth of a count of N: the one the autotuner measured
+    fastest for a whole count (load + run, autotune.tuned_preset), else 5 as
+    18Py always uses (:2139), up to N = 22.
 
     The walk keeps 16 parameter bytes per record, and a record of preset p
-    has its terminal row at depth N-1-p, so N = 23 starts from preset 6 and
-    N = 24 from preset 7 (where, unlike at preset 5, no record needs the P5
-    functionids)."""
-    return max(5, N - 17)
+    has its terminal row at depth N-1-p, so N = 23 needs preset 6 or more
+    and N = 24 preset 7 or more (where, unlike at preset 5, no record needs
+    the P5 functionids); a measured preset outside 4..7 or below N-17 is
+    not used."""
+    from .autotune import tuned_preset
+    lo = max(5, N - 17)
+    p = tuned_preset(N)
+    if p is not None and max(4, N - 17) <= p <= 7:
+        return p
+    return lo
 
 
 def constellations_for(N: int, preset: int, ctx: Context | None = None):

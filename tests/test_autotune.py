@@ -1,0 +1,84 @@
+"""Autotuner tables: the preset depth chosen per N and the schedule per
+(N, preset), written and read back without a GPU."""
+import json
+
+import pytest
+
+from nqueens import autotune, solver
+
+
+def _rows():
+    # two presets of N=20 as sweep_presets reports them (times in ms)
+    return [
+        {"preset": 4, "error": "nq_ctx_load: P5 functionid"},
+        {"preset": 5, "records": 139232, "load_ms": 40.0, "split_depth": 3, "block": 128, "lds_stack": 1,
+         "chunk": 64, "items": 1, "kernel_ms_median": 500.0, "kernel_ms_best": 499.0,
+         "run_ms_median": 520.0, "run_ms_best": 519.0},
+        {"preset": 5, "records": 139232, "load_ms": 40.0, "split_depth": 4, "block": 256, "lds_stack": 1,
+         "chunk": 64, "items": 1, "kernel_ms_median": 530.0, "kernel_ms_best": 529.0,
+         "run_ms_median": 550.0, "run_ms_best": 549.0},
+        {"preset": 6, "records": 1500000, "load_ms": 60.0, "split_depth": 2, "block": 256, "lds_stack": 1,
+         "chunk": 64, "items": 1, "kernel_ms_median": 480.0, "kernel_ms_best": 478.0,
+         "run_ms_median": 490.0, "run_ms_best": 488.0},
+    ]
+
+
+@pytest.fixture
+def tables(tmp_path, monkeypatch):
+    """The autotuner's tables redirected to empty files under tmp_path."""
+    tp, pp = tmp_path / "tuned.json", tmp_path / "tuned_presets.json"
+    monkeypatch.setattr(autotune, "TUNED_PATH", tp)
+    monkeypatch.setattr(autotune, "PRESETS_PATH", pp)
+    monkeypatch.setattr(autotune, "TUNED", {})
+    monkeypatch.setattr(autotune, "TUNED_PRESETS", {})
+    return tp, pp
+
+
+def test_record_presets_picks_fastest_whole_count(tables):
+    tp, pp = tables
+    best = autotune.record_presets(20, _rows(), path=pp, tuned_path=tp)
+    # preset 6: 60 + 490 < preset 5: 40 + 520
+    assert best["preset"] == 6
+    assert best["per_preset"]["5"]["split_depth"] == 3 and best["per_preset"]["5"]["block"] == 128
+    assert json.loads(pp.read_text())["20"]["preset"] == 6
+    t = json.loads(tp.read_text())
+    assert t["20,5"]["split_depth"] == 3 and t["20,6"]["split_depth"] == 2
+    assert autotune.tuned_preset(20) == 6 and autotune.tuned_preset(19) is None
+    assert solver.preset_for(20) == 6
+    s = autotune.best_sched(20, 6)
+    assert (s.split_depth, s.block, s.lds_stack) == (2, 256, 1)
+
+
+def test_record_presets_load_time_counts(tables):
+    tp, pp = tables
+    rows = _rows()
+    rows[3]["load_ms"] = 100.0      # preset 6 now loses: 100 + 490 > 40 + 520
+    assert autotune.record_presets(20, rows, path=pp, tuned_path=tp)["preset"] == 5
+
+
+def test_record_presets_all_failed(tables):
+    tp, pp = tables
+    with pytest.raises(ValueError):
+        autotune.record_presets(20, [_rows()[0]], path=pp, tuned_path=tp)
+
+
+def test_preset_for_ignores_unusable_measurements(monkeypatch):
+    # a table entry the walk cannot take (terminal row deeper than 16, or
+    # outside 4..7) falls back to the rule
+    monkeypatch.setattr(autotune, "TUNED_PRESETS", {"23": {"preset": 5}, "12": {"preset": 3},
+                                                    "13": {"preset": 8}, "14": {"preset": 4}})
+    assert solver.preset_for(23) == 6
+    assert solver.preset_for(12) == 5
+    assert solver.preset_for(13) == 5
+    assert solver.preset_for(14) == 4
+    assert solver.preset_for(24) == 7
+
+
+def test_shipped_preset_table_is_usable():
+    """Every preset the shipped table holds keeps the terminal row within the
+    16 parameter bytes, and its schedule is in the schedule table."""
+    for n, e in autotune.TUNED_PRESETS.items():
+        N, p = int(n), int(e["preset"])
+        assert max(4, N - 17) <= p <= 7
+        assert solver.preset_for(N) == p
+        assert f"{N},{p}" in autotune.TUNED

@@ -330,3 +330,25 @@ def test_trained_model_gains_over_fifo(ctx, oracle):
     print(f"makespan / work bound: FIFO {fifo:.3f}, model order {model:.3f}, true order {best:.3f}")
     assert best <= model < fifo
     assert model < 1.25 and fifo > 1.6
+
+
+def test_autotune_preset_sweep(ctx, known, tmp_path, monkeypatch):
+    """The preset sweep counts N=13 at presets 4..7 (host generation up to 5,
+    device expansion beyond), checks every total, and stores the fastest
+    whole count; the records of each preset are the generator's."""
+    from nqueens import autotune
+    tp, pp = tmp_path / "tuned.json", tmp_path / "tuned_presets.json"
+    monkeypatch.setattr(autotune, "TUNED", {})
+    monkeypatch.setattr(autotune, "TUNED_PRESETS", {})
+    N = 13
+    rows = autotune.sweep_presets(ctx, N, known["total_a000170"][str(N)], presets=(4, 5, 6, 7), splits=(2, 3),
+                                  blocks=(64,), reps=1, warmup=0)
+    assert all("error" not in r for r in rows), rows
+    for p in (4, 5, 6, 7):
+        rp = [r for r in rows if r["preset"] == p]
+        assert len(rp) == 2 and all(r["records"] == C.gen_constellations(N, p).shape[0] for r in rp)
+    best = autotune.record_presets(N, rows, path=pp, tuned_path=tp)
+    assert best["preset"] in (4, 5, 6, 7)
+    assert nqueens.solver.preset_for(N) == best["preset"]
+    total, unique = nqueens.count(N, ctx=ctx)
+    assert (total, unique) == tuple(known["readme_total_unique"][str(N)])

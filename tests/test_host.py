@@ -124,7 +124,7 @@ def test_solver_argument_checks():
     with pytest.raises(ValueError):
         nqueens.count(16, chunks=0)
     # terminal depth N-1-preset stays within the walk's 16 parameter bytes
-    assert [preset_for(n) for n in (19, 22, 23, 24)] == [5, 5, 6, 7]
+    assert [preset_for(n) for n in (22, 23, 24)] == [5, 6, 7]    # no measured preset: the rule
     assert all(n - 1 - preset_for(n) <= 16 for n in range(6, 25))
 
 
