@@ -28,7 +28,7 @@ import numpy as np
 
 __all__ = [
     "to_ijkl", "geti", "getj", "getk", "getl", "rot90", "rot180", "mirvert",
-    "symmetry", "symmetry90", "jasmin", "check_rotations", "start_constellations",
+    "symmetry", "symmetry90", "jasmin", "check_rotations", "start_constellations", "start_array",
     "gen_constellations", "read_constellations_bin", "write_constellations_bin",
     "read_solutions_bin", "write_solutions_bin", "records_sorted",
 ]
@@ -109,42 +109,75 @@ def check_rotations(seen: set, i: int, j: int, k: int, l: int, N: int) -> bool:
         ((l << 15) + (k << 10) + ((N - 1 - i) << 5) + (N - 1 - j)) in seen
 
 
+def _grid(*ranges) -> list[np.ndarray]:
+    """Index arrays of nested loops over `ranges` (first range outermost), in
+    loop order."""
+    g = np.meshgrid(*[np.asarray(r, np.int64) for r in ranges], indexing="ij")
+    return [a.ravel() for a in g]
+
+
+def _jasmin_array(x: np.ndarray, N: int) -> np.ndarray:
+    """`jasmin` of every element of x (same comparisons, same tie order)."""
+    N1 = N - 1
+    i, j, k, l = (x >> 15) & 31, (x >> 10) & 31, (x >> 5) & 31, x & 31
+    arg = np.zeros_like(x)
+    mn = np.minimum(j, N1 - j)
+    for a, v in ((2, i), (3, k), (1, l)):
+        d = np.minimum(v, N1 - v)
+        t = d < mn
+        arg = np.where(t, a, arg)
+        mn = np.where(t, d, mn)
+    for r in range(3):                                    # arg quarter turns
+        t = arg > r
+        i, j, k, l = (np.where(t, N1 - k, i), np.where(t, N1 - l, j), np.where(t, j, k), np.where(t, i, l))
+    mv = j < N1 - j                                       # mirvert
+    return (np.where(mv, N1 - i, i) << 15) | (np.where(mv, N1 - j, j) << 10) | (np.where(mv, l, k) << 5) | \
+        np.where(mv, k, l)
+
+
+def _first_occurrences(key: np.ndarray) -> np.ndarray:
+    """Positions of the first occurrence of each distinct key, ascending."""
+    _, first = np.unique(key, return_index=True)
+    return np.sort(first)
+
+
+def start_array(N: int) -> np.ndarray:
+    """`start_constellations` as an int64 array."""
+    N1, N2, half = N - 1, N - 2, (N + 1) // 2
+    blocks = []
+    centre = [N // 2] if N % 2 == 1 else []               # centre block, 18Py:1687-1699
+    for k in centre + list(range(1, half)):               # no corner queen, 18Py:1703-1712
+        l, i, j = _grid(range(k + 1, N1), range(k + 1, N1), range(N - k - 2, 0, -1))
+        m = (i != N1 - l) & (j != i) & (j != l)
+        blocks.append((i[m], j[m], np.full(int(m.sum()), k, np.int64), l[m]))
+    if not blocks:
+        return np.zeros(0, np.int64)
+    i, j, k, l = (np.concatenate(a) for a in zip(*blocks))
+    x = (i << 15) | (j << 10) | (k << 5) | l
+    # a candidate is skipped when a quarter, half or three-quarter turn of it
+    # was added before it; the turns form orbits, so exactly the first
+    # candidate of each orbit (in loop order) is added
+    r90 = ((N1 - k) << 15) | ((N1 - l) << 10) | (j << 5) | i
+    r180 = ((N1 - j) << 15) | ((N1 - i) << 10) | ((N1 - l) << 5) | (N1 - k)
+    r270 = (l << 15) | (k << 10) | ((N1 - i) << 5) | (N1 - j)
+    orbit = np.minimum(np.minimum(x, r90), np.minimum(r180, r270))
+    kept = x[_first_occurrences(orbit)]
+    j, l = _grid(range(1, N2), range(0, N1))              # corner queen, 18Py:1716 (no check)
+    m = l > j
+    seen = np.concatenate([kept, (j[m] << 10) | l[m]])
+    jas = _jasmin_array(seen, N)                          # 18Py:1719, first occurrence kept
+    return jas[_first_occurrences(jas)]
+
+
 def start_constellations(N: int) -> list[int]:
     """Canonical start constellations (Jasmin forms), in insertion order.
 
     Candidates are added one at a time, each skipped when one of its rotations
     was added before it -- the incremental ``set.update`` of 18Py:1687-1712 --
-    then every survivor is mapped to its Jasmin form (18Py:1719)."""
-    seen: dict[int, None] = {}
-    N1, N2, half = N - 1, N - 2, (N + 1) // 2
-    if N % 2 == 1:                                        # centre block, 18Py:1687-1699
-        c = N // 2
-        for l in range(c + 1, N1):
-            for i in range(c + 1, N1):
-                if i == N1 - l:
-                    continue
-                for j in range(N - c - 2, 0, -1):
-                    if j == i or j == l:
-                        continue
-                    if not check_rotations(seen, i, j, c, l, N):
-                        seen.setdefault(to_ijkl(i, j, c, l))
-    for k in range(1, half):                              # no corner queen, 18Py:1703-1712
-        for l in range(k + 1, N1):
-            for i in range(k + 1, N1):
-                if i == N1 - l:
-                    continue
-                for j in range(N - k - 2, 0, -1):
-                    if j == i or j == l:
-                        continue
-                    if not check_rotations(seen, i, j, k, l, N):
-                        seen.setdefault(to_ijkl(i, j, k, l))
-    for j in range(1, N2):                                # corner queen, 18Py:1716
-        for l in range(j + 1, N1):
-            seen.setdefault(to_ijkl(0, j, 0, l))
-    out: dict[int, None] = {}
-    for x in seen:
-        out.setdefault(jasmin(x, N))
-    return list(out)
+    then every survivor is mapped to its Jasmin form (18Py:1719).  Computed
+    over arrays: within each rotation orbit the first candidate in loop order
+    is the one added (`start_array`)."""
+    return start_array(N).tolist()
 
 
 def gen_constellations(N: int, preset: int = 5) -> np.ndarray:
@@ -161,7 +194,7 @@ def gen_constellations(N: int, preset: int = 5) -> np.ndarray:
         raise ValueError(f"N={N} outside 5..27")
     if not 3 <= preset <= 10:
         raise ValueError(f"preset={preset} outside 3..10")
-    scs = np.array(start_constellations(N), dtype=np.int64)
+    scs = start_array(N)
     if scs.size == 0:
         return np.zeros((0, 4), np.uint32)
     i, j, k, l = (scs >> 15) & 31, (scs >> 10) & 31, (scs >> 5) & 31, scs & 31

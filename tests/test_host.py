@@ -27,6 +27,20 @@ def test_generator_matches_oracle(oracle, N, preset):
     assert records_sha256(mine) == records_sha256(theirs)
 
 
+@pytest.mark.parametrize("N", [8, 11, 12, 17, 20, 21])
+def test_start_order_matches_oracle(oracle, N):
+    """The start constellations come in the reference's insertion order (the
+    oracle's loop restatement of 18Py:1687-1719): the order in which the
+    oracle's records first use each start."""
+    theirs, _ = oracle.gen_constellations(N, 5, scope=oracle.SCOPE_NONE)
+    ijkl = theirs[:, 3] & 0xFFFFF
+    _, first = np.unique(ijkl, return_index=True)
+    order = ijkl[np.sort(first)].tolist()
+    used = set(order)
+    assert [x for x in C.start_constellations(N) if x in used] == order
+    assert len(C.start_constellations(N)) == len(set(C.start_constellations(N)))
+
+
 def test_generator_preset_counts(known):
     pc = known["preset_record_counts"]
     assert C.gen_constellations(18, 6).shape[0] == pc["18,6"]
