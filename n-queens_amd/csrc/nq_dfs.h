@@ -230,14 +230,17 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     uint32_t nf = bop3<kAndNotOr>(L.mask, occ, ncol);
 #if defined(__HIP_DEVICE_COMPILE__)
     asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
-#endif
-    const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);   // all ones when the frame has no prune
-#if defined(__HIP_DEVICE_COMPILE__)
+    // all ones when the frame has no prune: C << 1 as an add, then the sign
+    // (two full-rate ops measured faster than one v_bfe_i32 here)
+    uint32_t c2;
+    asm("v_add_u32 %0, %1, %1" : "=v"(c2) : "v"(C));
+    const uint32_t nla = (uint32_t)((int32_t)c2 >> 31);
     uint32_t nld2;
     // nld << 1 as an add: v_add_u32 issues at the full rate, v_lshlrev_b32
     // at half (the compiler turns x + x back into a shift)
     asm("v_add_u32 %0, %1, %1" : "=v"(nld2) : "v"(nld));
 #else
+    const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);
     const uint32_t nld2 = nld << 1;
 #endif
     const uint32_t look = bop3<kOr3>(nld2, nrd >> 1, ncol);
@@ -260,7 +263,14 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const bool pop = up & nonempty;                      // ... and drop it from the stack
     const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
     const uint32_t A = w.A;
-    const uint32_t Wc = hiW + nf - (1u << kRShift);      // the child: r - 1, candidates nf
+#if defined(__HIP_DEVICE_COMPILE__)
+    // the child: r - 1, candidates nf; two two-operand adds (the compiler
+    // would fuse them into a v_add3 with a scalar operand, which issues slower)
+    uint32_t Wc;
+    asm("v_add_u32 %0, %1, %2" : "=v"(Wc) : "v"(hiW - (1u << kRShift)), "v"(nf));
+#else
+    const uint32_t Wc = hiW + nf - (1u << kRShift);
+#endif
     const uint32_t colc = bop3<kSel>(L.cm, ncol, qc);    // columns from ncol, parameter byte from qc
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
