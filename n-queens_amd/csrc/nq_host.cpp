@@ -136,6 +136,31 @@ int nq_abi_version(void) { return NQ_ABI_VERSION; }
 
 const char *nq_last_error(void) { return g_err.c_str(); }
 
+namespace {
+
+// Device properties, stream, events and the small fixed buffers of a new
+// context (the device is current).
+int init_ctx(nq_ctx *c)
+{
+    hipDeviceProp_t prop;
+    NQ_HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    c->cu = prop.multiProcessorCount;
+    c->lds_max = (int)prop.sharedMemPerBlock;
+    c->wave = prop.warpSize;
+    c->clock_khz = prop.clockRate;
+    if (c->wave != 64) return nq_fail(NQ_EUNSUPPORTED, "wavefront size %d, kernels assume 64", prop.warpSize);
+    NQ_HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) NQ_HIPCHK(hipEventCreate(&e));
+    NQ_HIPCHK(c->total.reserve(1));
+    NQ_HIPCHK(c->err.reserve(1));
+    NQ_HIPCHK(c->head.reserve(1));
+    NQ_HIPCHK(c->item_count.reserve(1));
+    NQ_HIPCHK(c->sym.reserve(2));
+    return 0;
+}
+
+}  // namespace
+
 int nq_ctx_create(int device, nq_ctx **out)
 {
     if (!out) return nq_fail(NQ_EINVAL, "null out pointer");
@@ -146,20 +171,10 @@ int nq_ctx_create(int device, nq_ctx **out)
     NQ_HIPCHK(hipSetDevice(device));
     nq_ctx *c = new nq_ctx;
     c->device = device;
-    hipDeviceProp_t prop;
-    NQ_HIPCHK(hipGetDeviceProperties(&prop, device));
-    c->cu = prop.multiProcessorCount;
-    c->lds_max = (int)prop.sharedMemPerBlock;
-    c->wave = prop.warpSize;
-    c->clock_khz = prop.clockRate;
-    if (c->wave != 64) { delete c; return nq_fail(NQ_EUNSUPPORTED, "wavefront size %d, kernels assume 64", prop.warpSize); }
-    NQ_HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto &e : c->ev) NQ_HIPCHK(hipEventCreate(&e));
-    NQ_HIPCHK(c->total.reserve(1));
-    NQ_HIPCHK(c->err.reserve(1));
-    NQ_HIPCHK(c->head.reserve(1));
-    NQ_HIPCHK(c->item_count.reserve(1));
-    NQ_HIPCHK(c->sym.reserve(2));
+    if (int r = init_ctx(c)) {
+        nq_ctx_destroy(c);   // releases whatever was created; keeps the message of the failure
+        return r;
+    }
     *out = c;
     return 0;
 }
