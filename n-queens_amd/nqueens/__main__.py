@@ -73,6 +73,11 @@ def chunked(a) -> int:
     first, last = a.chunk_range if a.chunk_range is not None else (0, a.chunks)
     preset = a.preset or solver.preset_for(N)
     log = ChunkLog(a.progress_file, N, a.chunks, preset) if a.progress_file else None
+    if log and a.preset is None and log.other_presets and not log.totals:
+        # the measured preset of N changed since the log was started: its
+        # chunks partition another record set
+        raise SystemExit(f"{a.progress_file} holds chunks of N={N} / {a.chunks} at preset "
+                         f"{sorted(log.other_presets)}, not at {preset}: pass --preset to resume it")
     todo = log.missing(first, last) if log else list(range(first, last))
     t0 = time.perf_counter()
     ctx = solver.default_context()

@@ -24,6 +24,7 @@ class ChunkLog:
         self.path = Path(path)
         self.key = (int(N), int(chunks), int(preset))
         self.totals: dict[int, int] = {}
+        self.other_presets: set[int] = set()   # presets of lines for the same N and chunk count
         if self.path.exists():
             self._read()
 
@@ -40,6 +41,8 @@ class ChunkLog:
             except ValueError:
                 raise ValueError(f"{self.path}:{no}: not a progress line: {line!r}") from None
             if (n, k, p) != self.key:
+                if (n, k) == self.key[:2]:
+                    self.other_presets.add(p)
                 continue
             if not 0 <= i < k or t < 0:
                 raise ValueError(f"{self.path}:{no}: chunk {i} of {k}, total {t}")

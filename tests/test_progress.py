@@ -37,3 +37,23 @@ def test_log_refuses_bad_lines(tmp_path):
         log.record(0, 6, 0.1)
     with pytest.raises(ValueError):
         log.total()
+
+
+def test_log_notes_other_presets(tmp_path):
+    """Lines of the same N and chunk count at another preset are not this
+    count's chunks, but the log says they are there (the CLI then refuses to
+    start over silently)."""
+    p = tmp_path / "run.tsv"
+    ChunkLog(p, 20, 4, 5).record(0, 100, 1.0)
+    log = ChunkLog(p, 20, 4, 4)
+    assert log.missing() == [0, 1, 2, 3] and log.other_presets == {5}
+    assert ChunkLog(p, 20, 8, 4).other_presets == set()
+
+
+def test_cli_refuses_log_of_another_preset(tmp_path):
+    from nqueens.__main__ import main
+    p = tmp_path / "run.tsv"
+    ChunkLog(p, 20, 4, 7).record(0, 100, 1.0)     # a preset the table never picks for N=20
+    with pytest.raises(SystemExit) as e:
+        main(["20", "--chunks", "4", "--progress-file", str(p)])
+    assert "pass --preset" in str(e.value)
