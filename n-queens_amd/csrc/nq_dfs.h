@@ -214,7 +214,11 @@ struct Walk {
 // with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots
 // above the inert frame's.  Returns true when the walk's subtree is
 // exhausted (the step took the inert frame).
-template <bool NODES, class Count, class Stack>
+// ISSUE: spend two full-rate VALU ops where one slower op would do (the
+// no-prune sign, the child's frame word).  That pays where the SIMD is
+// issue-bound (the persistent kernel, 16 waves per CU) and costs where a
+// lane's dependency chain is the limit (one lane per constellation).
+template <bool NODES, bool ISSUE = true, class Count, class Stack>
 NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint64_t &nodes)
 {
     const uint32_t S = st.S;
@@ -230,11 +234,15 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     uint32_t nf = bop3<kAndNotOr>(L.mask, occ, ncol);
 #if defined(__HIP_DEVICE_COMPILE__)
     asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
-    // all ones when the frame has no prune: C << 1 as an add, then the sign
-    // (two full-rate ops measured faster than one v_bfe_i32 here)
-    uint32_t c2;
-    asm("v_add_u32 %0, %1, %1" : "=v"(c2) : "v"(C));
-    const uint32_t nla = (uint32_t)((int32_t)c2 >> 31);
+    // all ones when the frame has no prune
+    uint32_t nla;
+    if constexpr (ISSUE) {   // C << 1 as an add, then the sign (not one v_bfe_i32)
+        uint32_t c2;
+        asm("v_add_u32 %0, %1, %1" : "=v"(c2) : "v"(C));
+        nla = (uint32_t)((int32_t)c2 >> 31);
+    } else {
+        nla = (uint32_t)((int32_t)(C << 1) >> 31);
+    }
     uint32_t nld2;
     // nld << 1 as an add: v_add_u32 issues at the full rate, v_lshlrev_b32
     // at half (the compiler turns x + x back into a shift)
@@ -263,13 +271,11 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const bool pop = up & nonempty;                      // ... and drop it from the stack
     const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
     const uint32_t A = w.A;
+    // the child: r - 1, candidates nf
+    uint32_t Wc = hiW + nf - (1u << kRShift);
 #if defined(__HIP_DEVICE_COMPILE__)
-    // the child: r - 1, candidates nf; two two-operand adds (the compiler
-    // would fuse them into a v_add3 with a scalar operand, which issues slower)
-    uint32_t Wc;
-    asm("v_add_u32 %0, %1, %2" : "=v"(Wc) : "v"(hiW - (1u << kRShift)), "v"(nf));
-#else
-    const uint32_t Wc = hiW + nf - (1u << kRShift);
+    if constexpr (ISSUE)   // two two-operand adds, not a v_add3 with a scalar operand
+        asm("v_add_u32 %0, %1, %2" : "=v"(Wc) : "v"(hiW - (1u << kRShift)), "v"(nf));
 #endif
     const uint32_t colc = bop3<kSel>(L.cm, ncol, qc);    // columns from ncol, parameter byte from qc
     w.ld = down ? nld : (up ? top.x : w.ld);

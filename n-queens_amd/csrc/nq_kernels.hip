@@ -34,7 +34,7 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
     Walk w{root.x, root.y, start_col(S, root.z, 0u), start_word(S, root.w, 0u), st.empty()};
-    while (!walk_step<NODES>(L, w, st, cnt, nodes)) {
+    while (!walk_step<NODES, false>(L, w, st, cnt, nodes)) {   // one lane per record: latency-bound
     }
 }
 
