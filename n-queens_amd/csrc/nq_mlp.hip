@@ -114,80 +114,177 @@ __global__ void k_mlp_forward(const float *__restrict__ params, const float *__r
     y[b] = out;
 }
 
-// per-row backward of the loss mean((y - t)^2): dz3, dz2 and dz1 (pre-ReLU grads)
-__global__ void k_mlp_backward_rows(const float *__restrict__ params, const float *__restrict__ target,
-                                    const uint32_t *__restrict__ rows, uint32_t m, float inv_m,
-                                    const float *__restrict__ y, const float *__restrict__ h1s,
-                                    const float *__restrict__ h2s, float *__restrict__ dz1s,
-                                    float *__restrict__ dz2s, float *__restrict__ dz3s, float *__restrict__ loss)
+// ---------------------------------------------------------------------------
+// training step: rows kernel (forward + per-row backward), weight-gradient
+// partials over row chunks, fixed-order sum of the partials, Adam
+// ---------------------------------------------------------------------------
+constexpr int kRowsPerBlock = 4;      // one wave per row, one lane per hidden unit
+constexpr int kP1 = kIn + 1, kP2 = kH + 1;   // padded LDS row strides: a lane per row, distinct banks
+constexpr int kChunk = 64;            // rows per weight-gradient chunk
+constexpr int kMaxParts = 64;         // weight-gradient blocks (partials) per parameter group
+
+__device__ __forceinline__ float wave_sum_f32(float v)
 {
-    __shared__ float sp[kNP];
-    __shared__ float red[256];
-    stage_params(params, sp);
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    float l = 0.f;
-    if (b < m) {
-        const uint32_t src = rows ? rows[b] : b;
-        const float e = y[b] - target[src];
-        l = e * e;
-        const float dz3 = 2.f * e * inv_m;
-        dz3s[b] = dz3;
-        float dz2[kH];
-#pragma unroll 4
-        for (int o = 0; o < kH; o++) {
-            const float h = h2s[(size_t)b * kH + o];
-            dz2[o] = h > 0.f ? dz3 * sp[kW3 + o] : 0.f;
-            dz2s[(size_t)b * kH + o] = dz2[o];
-        }
-        for (int i = 0; i < kH; i++) {
-            const float h = h1s[(size_t)b * kH + i];
-            float g = 0.f;
-            if (h > 0.f) {
-#pragma unroll 8
-                for (int o = 0; o < kH; o++) g = fmaf(sp[kW2 + o * kH + i], dz2[o], g);
-            }
-            dz1s[(size_t)b * kH + i] = g;
-        }
-    }
-    red[threadIdx.x] = l;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_m);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
-// weight gradients: one thread per parameter, summing over the m rows
-__global__ void k_mlp_wgrad(const float *__restrict__ x, const uint32_t *__restrict__ rows, uint32_t m,
-                            const float *__restrict__ h1s, const float *__restrict__ h2s,
-                            const float *__restrict__ dz1s, const float *__restrict__ dz2s,
-                            const float *__restrict__ dz3s, float *__restrict__ grad)
+// Forward and per-row backward of the loss mean((y - t)^2) over rows
+// b = 0..m-1 (row b is x[rows[b]], or x[b] without rows): stores the
+// activations h1, h2 and the pre-ReLU gradients dz1, dz2, dz3 of each row for
+// the weight gradient, the prediction, and adds the rows' loss * inv_m to
+// *loss.  Wave r of the block takes row blockIdx.x * 4 + r; lane o computes
+// hidden unit o of both layers (weights staged in LDS with padded rows, so a
+// wave reads 64 rows of W at one column without bank conflicts).  Waves past
+// the last row keep to the barriers and store nothing.
+__global__ void __launch_bounds__(256) k_mlp_rows(const float *__restrict__ params, const float *__restrict__ x,
+                                                  const float *__restrict__ target,
+                                                  const uint32_t *__restrict__ rows, uint32_t m, float inv_m,
+                                                  float *__restrict__ pred, float *__restrict__ h1s,
+                                                  float *__restrict__ h2s, float *__restrict__ dz1s,
+                                                  float *__restrict__ dz2s, float *__restrict__ dz3s,
+                                                  float *__restrict__ loss)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kNP) return;
+    __shared__ float w1[kH * kP1];
+    __shared__ float w2[kH * kP2];
+    __shared__ float xs[kRowsPerBlock][kIn];
+    __shared__ float hs[kRowsPerBlock][kH];
+    __shared__ float ds[kRowsPerBlock][kH];
+    __shared__ float ls[kRowsPerBlock];
+    for (int a = threadIdx.x; a < kH * kIn; a += blockDim.x) w1[(a / kIn) * kP1 + a % kIn] = params[kW1 + a];
+    for (int a = threadIdx.x; a < kH * kH; a += blockDim.x) w2[(a / kH) * kP2 + a % kH] = params[kW2 + a];
+    const int r = threadIdx.x >> 6, o = threadIdx.x & 63;
+    const uint32_t b = blockIdx.x * kRowsPerBlock + r;
+    const bool live = b < m;
+    const uint32_t src = live ? (rows ? rows[b] : b) : 0u;
+    if (o < kIn) xs[r][o] = live ? x[(size_t)src * kIn + o] : 0.f;
+    __syncthreads();
+    float z = params[kB1 + o];
+#pragma unroll
+    for (int i = 0; i < kIn; i++) z = fmaf(w1[o * kP1 + i], xs[r][i], z);
+    const float h1 = fmaxf(z, 0.f);
+    hs[r][o] = h1;
+    __syncthreads();
+    z = params[kB2 + o];
+#pragma unroll 16
+    for (int i = 0; i < kH; i++) z = fmaf(w2[o * kP2 + i], hs[r][i], z);
+    const float h2 = fmaxf(z, 0.f);
+    const float w3 = params[kW3 + o];
+    const float y = wave_sum_f32(w3 * h2) + params[kB3];
+    const float e = y - (live ? target[src] : y);
+    const float dz3 = 2.f * e * inv_m;
+    const float dz2 = h2 > 0.f ? dz3 * w3 : 0.f;
+    ds[r][o] = dz2;
+    if (o == 0) ls[r] = e * e;
+    __syncthreads();
     float g = 0.f;
-    if (p < kB1) {                       // W1[o][i]
-        const int o = p / kIn, i = p % kIn;
-        for (uint32_t b = 0; b < m; b++) {
-            const uint32_t src = rows ? rows[b] : b;
-            g = fmaf(dz1s[(size_t)b * kH + o], x[(size_t)src * kIn + i], g);
-        }
-    } else if (p < kW2) {                // b1[o]
-        const int o = p - kB1;
-        for (uint32_t b = 0; b < m; b++) g += dz1s[(size_t)b * kH + o];
-    } else if (p < kB2) {                // W2[o][i]
-        const int q = p - kW2, o = q / kH, i = q % kH;
-        for (uint32_t b = 0; b < m; b++) g = fmaf(dz2s[(size_t)b * kH + o], h1s[(size_t)b * kH + i], g);
-    } else if (p < kW3) {                // b2[o]
-        const int o = p - kB2;
-        for (uint32_t b = 0; b < m; b++) g += dz2s[(size_t)b * kH + o];
-    } else if (p < kB3) {                // W3[i]
-        const int i = p - kW3;
-        for (uint32_t b = 0; b < m; b++) g = fmaf(dz3s[b], h2s[(size_t)b * kH + i], g);
-    } else {                             // b3
-        for (uint32_t b = 0; b < m; b++) g += dz3s[b];
+    if (h1 > 0.f) {
+#pragma unroll 16
+        for (int q = 0; q < kH; q++) g = fmaf(w2[q * kP2 + o], ds[r][q], g);
     }
+    if (live) {
+        const size_t row = (size_t)b * kH + o;
+        h1s[row] = h1;
+        h2s[row] = h2;
+        dz2s[row] = dz2;
+        dz1s[row] = g;
+        if (o == 0) {
+            pred[b] = y;
+            dz3s[b] = dz3;
+        }
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, (ls[0] + ls[1] + ls[2] + ls[3]) * inv_m);
+}
+
+// Weight-gradient partials.  blockIdx.y picks the parameter group (0: W1,
+// b1; 1: W2, b2; 2: W3, b3); block x of gridDim.x takes the row chunks
+// x, x + gridDim.x, ... (64 rows each, staged in LDS, rows past m as zeros)
+// and writes its sums to part[x][parameter].  Every output is summed in a
+// fixed order, so the gradient does not depend on scheduling.
+__global__ void __launch_bounds__(256) k_mlp_wgrad_part(const float *__restrict__ x,
+                                                        const uint32_t *__restrict__ rows, uint32_t m,
+                                                        const float *__restrict__ h1s,
+                                                        const float *__restrict__ h2s,
+                                                        const float *__restrict__ dz1s,
+                                                        const float *__restrict__ dz2s,
+                                                        const float *__restrict__ dz3s, float *__restrict__ part)
+{
+    __shared__ float A[kChunk][kH];       // dz1 (group 0), dz2 (group 1), h2 (group 2)
+    __shared__ float B[kChunk][kH + 1];   // x (group 0), h1 (group 1), dz3 in column 0 (group 2)
+    constexpr int kOutMax = (kH * kH + kH + 255) / 256;   // outputs per thread, group 1
+    const int grp = blockIdx.y;
+    const int nout = grp == 0 ? kH * kIn + kH : grp == 1 ? kH * kH + kH : kH + 1;
+    float acc[kOutMax];
+#pragma unroll
+    for (int k = 0; k < kOutMax; k++) acc[k] = 0.f;
+    const uint32_t nchunk = (m + kChunk - 1) / kChunk;
+    for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const uint32_t b0 = c * kChunk;
+        for (int a = threadIdx.x; a < kChunk * kH; a += blockDim.x) {
+            const int j = a / kH, k = a % kH;
+            const uint32_t b = b0 + j;
+            const bool ok = b < m;
+            const size_t row = (size_t)b * kH + k;
+            A[j][k] = ok ? (grp == 0 ? dz1s[row] : grp == 1 ? dz2s[row] : h2s[row]) : 0.f;
+            if (grp == 1) B[j][k] = ok ? h1s[row] : 0.f;
+        }
+        if (grp == 0) {
+            for (int a = threadIdx.x; a < kChunk * kIn; a += blockDim.x) {
+                const int j = a / kIn, i = a % kIn;
+                const uint32_t b = b0 + j;
+                B[j][i] = b < m ? x[(size_t)(rows ? rows[b] : b) * kIn + i] : 0.f;
+            }
+        } else if (grp == 2) {
+            for (int j = threadIdx.x; j < kChunk; j += blockDim.x) B[j][0] = b0 + j < m ? dz3s[b0 + j] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kOutMax; k++) {
+            const int q = threadIdx.x + 256 * k;
+            if (q >= nout) continue;
+            float g = acc[k];
+            if (grp == 0) {
+                if (q < kH * kIn) {
+                    const int o = q / kIn, i = q % kIn;
+                    for (int j = 0; j < kChunk; j++) g = fmaf(A[j][o], B[j][i], g);
+                } else {
+                    for (int j = 0; j < kChunk; j++) g += A[j][q - kH * kIn];
+                }
+            } else if (grp == 1) {
+                if (q < kH * kH) {
+                    const int o = q / kH, i = q % kH;
+                    for (int j = 0; j < kChunk; j++) g = fmaf(A[j][o], B[j][i], g);
+                } else {
+                    for (int j = 0; j < kChunk; j++) g += A[j][q - kH * kH];
+                }
+            } else {
+                if (q < kH) {
+                    for (int j = 0; j < kChunk; j++) g = fmaf(B[j][0], A[j][q], g);
+                } else {
+                    for (int j = 0; j < kChunk; j++) g += B[j][0];
+                }
+            }
+            acc[k] = g;
+        }
+        __syncthreads();
+    }
+    const int base = grp == 0 ? kW1 : grp == 1 ? kW2 : kW3;   // each group's parameters are contiguous
+    float *out = part + (size_t)blockIdx.x * kNP + base;
+#pragma unroll
+    for (int k = 0; k < kOutMax; k++) {
+        const int q = threadIdx.x + 256 * k;
+        if (q < nout) out[q] = acc[k];
+    }
+}
+
+// grad[p] = sum of the nparts partials of parameter p, in block order
+__global__ void k_mlp_grad_sum(const float *__restrict__ part, uint32_t nparts, float *__restrict__ grad)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kNP) return;
+    float g = 0.f;
+    for (uint32_t c = 0; c < nparts; c++) g += part[(size_t)c * kNP + p];
     grad[p] = g;
 }
 
@@ -237,7 +334,7 @@ __global__ void k_iota(uint32_t *__restrict__ p, uint32_t m)
 namespace {
 
 struct MlpBufs {
-    nqk_buf::Buf<float> params, grad, m1, m2, x, y, pred, h1, h2, dz1, dz2, dz3, loss;
+    nqk_buf::Buf<float> params, grad, m1, m2, x, y, pred, h1, h2, dz1, dz2, dz3, loss, part;
     nqk_buf::Buf<uint32_t> rows;
 };
 
@@ -313,18 +410,20 @@ int nq_mlp_forward(nq_ctx *c, const float *params, const float *x, uint64_t m, f
 
 namespace {
 
-// one minibatch: forward, per-row backward, weight gradient (into B.grad), loss added to B.loss
+// one minibatch: forward and per-row backward, weight-gradient partials, their
+// fixed-order sum (into B.grad); the loss is added to B.loss
 int batch_grad(hipStream_t s, MlpBufs &B, const uint32_t *rows, uint32_t nb, float inv_m)
 {
     using namespace nqk;
-    const uint32_t g = (nb + 255) / 256;
-    k_mlp_forward<<<g, 256, 0, s>>>(B.params.p, B.x.p, rows, nb, B.pred.p, B.h1.p, B.h2.p);
+    NQ_HIPCHK(B.part.reserve((size_t)kMaxParts * kNP));
+    k_mlp_rows<<<(nb + kRowsPerBlock - 1) / kRowsPerBlock, 64 * kRowsPerBlock, 0, s>>>(
+        B.params.p, B.x.p, B.y.p, rows, nb, inv_m, B.pred.p, B.h1.p, B.h2.p, B.dz1.p, B.dz2.p, B.dz3.p, B.loss.p);
     NQ_HIPCHK(hipGetLastError());
-    k_mlp_backward_rows<<<g, 256, 0, s>>>(B.params.p, B.y.p, rows, nb, inv_m, B.pred.p, B.h1.p, B.h2.p,
-                                          B.dz1.p, B.dz2.p, B.dz3.p, B.loss.p);
+    const uint32_t nparts = std::min<uint32_t>((nb + kChunk - 1) / kChunk, kMaxParts);
+    k_mlp_wgrad_part<<<dim3(nparts, 3), 256, 0, s>>>(B.x.p, rows, nb, B.h1.p, B.h2.p, B.dz1.p, B.dz2.p, B.dz3.p,
+                                                     B.part.p);
     NQ_HIPCHK(hipGetLastError());
-    k_mlp_wgrad<<<(kNP + 255) / 256, 256, 0, s>>>(B.x.p, rows, nb, B.h1.p, B.h2.p, B.dz1.p, B.dz2.p, B.dz3.p,
-                                                  B.grad.p);
+    k_mlp_grad_sum<<<(kNP + 255) / 256, 256, 0, s>>>(B.part.p, nparts, B.grad.p);
     NQ_HIPCHK(hipGetLastError());
     return 0;
 }
