@@ -51,7 +51,6 @@ hipError_t launch_symmetric(hipStream_t s, int N, unsigned long long *out2);
 // cost model (nq_mlp.hip)
 hipError_t launch_features(hipStream_t s, const uint4 *rec, const uint4 *root, const uint4 *sched,
                            uint32_t m, float *feat);
-hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x, uint32_t m, float *y,
-                              float *h1, float *h2);
+hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x, uint32_t m, float *y);
 
 }  // namespace nqk

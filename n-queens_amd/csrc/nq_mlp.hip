@@ -67,51 +67,48 @@ __global__ void k_features(const uint4 *__restrict__ rec, const uint4 *__restric
     f[23] = lsum * 0.1f;
 }
 
-// ---------------------------------------------------------------------------
-// forward: one thread per row; parameters staged in LDS
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void stage_params(const float *__restrict__ params, float *sp)
+__device__ __forceinline__ float wave_sum_f32(float v)
 {
-    for (int a = threadIdx.x; a < kNP; a += blockDim.x) sp[a] = params[a];
-    __syncthreads();
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
-// rows == nullptr: row b is x[b]; else x[rows[b]].  h1/h2 (may be null)
-// receive the post-ReLU activations of the m rows for the backward pass.
-__global__ void k_mlp_forward(const float *__restrict__ params, const float *__restrict__ x,
-                              const uint32_t *__restrict__ rows, uint32_t m, float *__restrict__ y,
-                              float *__restrict__ h1s, float *__restrict__ h2s)
+__device__ __forceinline__ float lane_f32(float v, int lane)
 {
-    __shared__ float sp[kNP];
-    stage_params(params, sp);
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m) return;
-    const uint32_t src = rows ? rows[b] : b;
-    float xin[kIn];
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// ---------------------------------------------------------------------------
+// inference: y[b] for rows b = 0..m-1.  Lane o of a wave holds hidden unit
+// o's weights in registers (row o of W1 and of W2, W3[o]) for the whole
+// kernel; the wave walks rows (wave w of the grid takes w, w + waves, ...),
+// broadcasting each row's features and first-layer activations by readlane.
+// Each hidden unit sums its inputs in index order; the output is a wave sum.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ params, const float *__restrict__ x,
+                                                   uint32_t m, float *__restrict__ y)
+{
+    const int o = threadIdx.x & 63;
+    float w1[kIn], w2[kH];
 #pragma unroll
-    for (int i = 0; i < kIn; i++) xin[i] = x[(size_t)src * kIn + i];
-    float h1[kH];
-#pragma unroll 4
-    for (int o = 0; o < kH; o++) {
-        float z = sp[kB1 + o];
+    for (int i = 0; i < kIn; i++) w1[i] = params[kW1 + o * kIn + i];
 #pragma unroll
-        for (int i = 0; i < kIn; i++) z = fmaf(sp[kW1 + o * kIn + i], xin[i], z);
-        h1[o] = fmaxf(z, 0.f);
+    for (int i = 0; i < kH; i++) w2[i] = params[kW2 + o * kH + i];
+    const float b1 = params[kB1 + o], b2 = params[kB2 + o], w3 = params[kW3 + o], b3 = params[kB3];
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < m; b += waves) {
+        const float xv = o < kIn ? x[(size_t)b * kIn + o] : 0.f;
+        float z = b1;
+#pragma unroll
+        for (int i = 0; i < kIn; i++) z = fmaf(w1[i], lane_f32(xv, i), z);
+        const float h1 = fmaxf(z, 0.f);
+        z = b2;
+#pragma unroll
+        for (int i = 0; i < kH; i++) z = fmaf(w2[i], lane_f32(h1, i), z);
+        const float out = wave_sum_f32(w3 * fmaxf(z, 0.f)) + b3;
+        if (o == 0) y[b] = out;
     }
-    float out = sp[kB3];
-    for (int o = 0; o < kH; o++) {
-        float z = sp[kB2 + o];
-#pragma unroll 8
-        for (int i = 0; i < kH; i++) z = fmaf(sp[kW2 + o * kH + i], h1[i], z);
-        const float h = fmaxf(z, 0.f);
-        if (h2s) h2s[(size_t)b * kH + o] = h;
-        out = fmaf(sp[kW3 + o], h, out);
-    }
-    if (h1s) {
-#pragma unroll 4
-        for (int o = 0; o < kH; o++) h1s[(size_t)b * kH + o] = h1[o];
-    }
-    y[b] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -122,13 +119,6 @@ constexpr int kRowsPerBlock = 4;      // one wave per row, one lane per hidden u
 constexpr int kP1 = kIn + 1, kP2 = kH + 1;   // padded LDS row strides: a lane per row, distinct banks
 constexpr int kChunk = 64;            // rows per weight-gradient chunk
 constexpr int kMaxParts = 64;         // weight-gradient blocks (partials) per parameter group
-
-__device__ __forceinline__ float wave_sum_f32(float v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // Forward and per-row backward of the loss mean((y - t)^2) over rows
 // b = 0..m-1 (row b is x[rows[b]], or x[b] without rows): stores the
@@ -310,11 +300,11 @@ hipError_t launch_features(hipStream_t s, const uint4 *rec, const uint4 *root, c
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x, uint32_t m, float *y,
-                              float *h1, float *h2)
+hipError_t launch_mlp_forward(hipStream_t s, const float *params, const float *x, uint32_t m, float *y)
 {
     if (m == 0) return hipSuccess;
-    k_mlp_forward<<<(m + 255) / 256, 256, 0, s>>>(params, x, nullptr, m, y, h1, h2);
+    const uint32_t blocks = std::min<uint32_t>((m + 3) / 4, 2048u);   // 4 waves per block, rows strided
+    k_mlp_infer<<<blocks, 256, 0, s>>>(params, x, m, y);
     return hipGetLastError();
 }
 
@@ -374,7 +364,7 @@ int nq_ctx_order_by_cost(nq_ctx *c, const float *params, uint32_t *order_out)
     NQ_HIPCHK(c->cm_idx.reserve(m));
     NQ_HIPCHK(hipMemcpyAsync(c->cm_params.p, params, nqk::kNP * sizeof(float), hipMemcpyHostToDevice, s));
     NQ_HIPCHK(nqk::launch_features(s, c->rec.p, c->root.p, c->sched.p, m, c->cm_x.p));
-    NQ_HIPCHK(nqk::launch_mlp_forward(s, c->cm_params.p, c->cm_x.p, m, c->cm_y.p, nullptr, nullptr));
+    NQ_HIPCHK(nqk::launch_mlp_forward(s, c->cm_params.p, c->cm_x.p, m, c->cm_y.p));
     nqk::k_iota<<<(m + 255) / 256, 256, 0, s>>>(c->cm_idx.p, m);
     NQ_HIPCHK(hipGetLastError());
     // predicted heaviest first; ties keep record order (the radix sort is stable)
@@ -402,7 +392,7 @@ int nq_mlp_forward(nq_ctx *c, const float *params, const float *x, uint64_t m, f
     NQ_HIPCHK(Y.reserve(m));
     NQ_HIPCHK(hipMemcpyAsync(P.p, params, nqk::kNP * sizeof(float), hipMemcpyHostToDevice, s));
     NQ_HIPCHK(hipMemcpyAsync(X.p, x, m * nqk::kIn * sizeof(float), hipMemcpyHostToDevice, s));
-    NQ_HIPCHK(nqk::launch_mlp_forward(s, P.p, X.p, (uint32_t)m, Y.p, nullptr, nullptr));
+    NQ_HIPCHK(nqk::launch_mlp_forward(s, P.p, X.p, (uint32_t)m, Y.p));
     NQ_HIPCHK(hipMemcpyAsync(y_out, Y.p, m * sizeof(float), hipMemcpyDeviceToHost, s));
     NQ_HIPCHK(hipStreamSynchronize(s));
     return 0;
