@@ -75,6 +75,7 @@ struct nq_ctx {
     bool has_splits = false;
     uint32_t split_min = 0;       // smallest per-record split depth
     nqk_buf::Buf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
+    nqk_buf::Buf<unsigned long long> scan_tmp;   // tile sums of the item-offset scan, (m + 1023) / 1024
     // cost-model ordering on the device (nq_ctx_order_by_cost)
     nqk_buf::Buf<float> cm_params, cm_x, cm_y, cm_ys;
     nqk_buf::Buf<uint32_t> cm_idx;

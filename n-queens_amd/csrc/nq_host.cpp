@@ -83,7 +83,7 @@ int size_items(nq_ctx *c, int split, int block)
                               c->order.p, split < 0 ? c->splits.p : nullptr, (uint32_t)c->m,
                               (uint32_t)(split < 0 ? 0 : split), c->nitems.p, nullptr, nullptr,
                               nullptr, nullptr));
-    NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p));
+    NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p, c->scan_tmp.p));
     unsigned long long cnt = 0;
     NQ_HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     NQ_HIPCHK(hipStreamSynchronize(c->stream));
@@ -114,7 +114,7 @@ int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool even
     const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
     NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
                               su, c->nitems.p, nullptr, nullptr, nullptr, nullptr));
-    NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p));
+    NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p, c->scan_tmp.p));
     NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
                               su, c->nitems.p, c->off.p, c->items.p,
                               c->raw.p, c->nodes.p));
@@ -207,16 +207,17 @@ namespace {
 int expand_device(nq_ctx *c, int N, nqk_buf::Buf<uint4> dev[2], int *cur, uint64_t *m_io, int levels)
 {
     nqk_buf::Buf<uint32_t> cnt, off;
-    nqk_buf::Buf<unsigned long long> total;
+    nqk_buf::Buf<unsigned long long> total, tsum;
     NQ_HIPCHK(total.reserve(1));
     uint64_t m = *m_io;
     int a = 0;
     for (int lv = 0; lv < levels && m; lv++) {
         NQ_HIPCHK(cnt.reserve(m));
         NQ_HIPCHK(off.reserve(m));
+        NQ_HIPCHK(tsum.reserve(m / 1024 + 1));
         NQ_HIPCHK(nqk::launch_fill_u32(c->stream, c->err.p, 1, 0));
         NQ_HIPCHK(nqk::launch_preset(c->stream, false, N, dev[a].p, (uint32_t)m, cnt.p, nullptr, nullptr, c->err.p));
-        NQ_HIPCHK(nqk::launch_scan(c->stream, cnt.p, (uint32_t)m, off.p, total.p));
+        NQ_HIPCHK(nqk::launch_scan(c->stream, cnt.p, (uint32_t)m, off.p, total.p, tsum.p));
         unsigned long long n = 0;
         uint32_t err = 0;
         NQ_HIPCHK(hipMemcpyAsync(&n, total.p, 8, hipMemcpyDeviceToHost, c->stream));
@@ -284,6 +285,7 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
     NQ_HIPCHK(c->order.reserve(mm));
     NQ_HIPCHK(c->nitems.reserve(mm));
     NQ_HIPCHK(c->off.reserve(mm));
+    NQ_HIPCHK(c->scan_tmp.reserve(mm / 1024 + 1));
     NQ_HIPCHK(c->weighted.reserve(mm));
     NQ_HIPCHK(c->nodes.reserve(mm));
     NQ_HIPCHK(c->raw.reserve(mm));

@@ -33,8 +33,10 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
                          uint32_t m, uint32_t split,
                          uint32_t *nitems, const uint32_t *off, uint4 *items,
                          unsigned long long *raw, unsigned long long *nodes_out);
-hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off,
-                       unsigned long long *count);
+// exclusive scan of nitems[0..m) into off (u32) and the total into *count;
+// tsum: scratch of (m + 1023) / 1024 u64 entries
+hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count,
+                       unsigned long long *tsum);
 hipError_t launch_preset(hipStream_t s, bool write, int N, const uint4 *in, uint32_t m, uint32_t *cnt,
                          const uint32_t *off, uint4 *out, uint32_t *err);
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,

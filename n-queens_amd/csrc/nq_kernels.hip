@@ -165,27 +165,69 @@ __global__ void k_preset(int N, const uint4 *__restrict__ in, uint32_t m, uint32
     if (!WRITE) cnt[c] = n;
 }
 
-// exclusive scan of nitems[0..m) -> off, total -> *count (single block of 1024)
-__global__ void k_scan(const uint32_t *__restrict__ nitems, uint32_t m, uint32_t *__restrict__ off,
-                       unsigned long long *__restrict__ count)
+// exclusive scan of nitems[0..m) -> off (mod 2^32, as the item indices), total
+// -> *count, in three launches: each 1024-entry tile scanned by one block
+// (coalesced; u64 sums), the tile sums scanned by one block, the tile offsets
+// added back
+constexpr uint32_t kScanTile = 1024;
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x, uint32_t lane)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_tile(const uint32_t *__restrict__ nitems, uint32_t m,
+                                                    uint32_t *__restrict__ off, unsigned long long *__restrict__ tsum)
+{
+    __shared__ uint64_t wsum[kScanTile / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint64_t a = (uint64_t)blockIdx.x * kScanTile + t;
+    const uint64_t v = a < m ? nitems[a] : 0u;
+    const uint64_t x = wave_incl_scan_u64(v, lane);
+    if (lane == 63u) wsum[w] = x;
+    __syncthreads();
+    uint64_t before = 0;
+    for (uint32_t k = 0; k < w; k++) before += wsum[k];
+    if (a < m) off[a] = (uint32_t)(before + x - v);
+    if (t == kScanTile - 1u) tsum[blockIdx.x] = before + x;
+}
+
+// exclusive scan of the n tile sums in place (one block of 1024), total -> *count
+__global__ void __launch_bounds__(1024) k_scan_tops(unsigned long long *__restrict__ tsum, uint32_t n,
+                                                    unsigned long long *__restrict__ count)
 {
     __shared__ uint64_t part[1024];
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (m + 1023u) / 1024u;
-    const uint32_t lo = t * per, hi = min(m, lo + per);
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t lo = min(n, t * per), hi = min(n, lo + per);
     uint64_t s = 0;
-    for (uint32_t a = lo; a < hi; a++) s += nitems[a];
+    for (uint32_t a = lo; a < hi; a++) s += tsum[a];
     part[t] = s;
     __syncthreads();
     for (uint32_t o = 1; o < 1024u; o <<= 1) {
-        uint64_t v = (t >= o) ? part[t - o] : 0;
+        const uint64_t v = t >= o ? part[t - o] : 0;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
     uint64_t run = part[t] - s;
-    for (uint32_t a = lo; a < hi; a++) { off[a] = (uint32_t)run; run += nitems[a]; }
+    for (uint32_t a = lo; a < hi; a++) {
+        const uint64_t v = tsum[a];
+        tsum[a] = run;
+        run += v;
+    }
     if (t == 1023u) *count = part[1023];
+}
+
+__global__ void k_scan_add(uint32_t *__restrict__ off, uint32_t m, const unsigned long long *__restrict__ tsum)
+{
+    const uint64_t a = (uint64_t)blockIdx.x * kScanTile + threadIdx.x;
+    if (a < m) off[a] += (uint32_t)tsum[blockIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -470,9 +512,17 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
     return hipGetLastError();
 }
 
-hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count)
+hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count,
+                       unsigned long long *tsum)
 {
-    k_scan<<<1, 1024, 0, s>>>(nitems, m, off, count);
+    const uint32_t tiles = (uint32_t)(((uint64_t)m + kScanTile - 1u) / kScanTile);
+    if (tiles) {
+        k_scan_tile<<<tiles, kScanTile, 0, s>>>(nitems, m, off, tsum);
+        if (hipError_t e = hipGetLastError()) return e;
+    }
+    k_scan_tops<<<1, 1024, 0, s>>>(tsum, tiles, count);
+    if (hipError_t e = hipGetLastError()) return e;
+    if (tiles) k_scan_add<<<tiles, kScanTile, 0, s>>>(off, m, tsum);
     return hipGetLastError();
 }
 
