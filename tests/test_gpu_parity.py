@@ -352,3 +352,19 @@ def test_autotune_preset_sweep(ctx, known, tmp_path, monkeypatch):
     assert nqueens.solver.preset_for(N) == best["preset"]
     total, unique = nqueens.count(N, ctx=ctx)
     assert (total, unique) == tuple(known["readme_total_unique"][str(N)])
+
+
+@pytest.mark.parametrize("m", [1, 1023, 1024, 1025, 2049])
+def test_batched_scan_tile_boundaries(ctx, oracle, m):
+    """Record counts on either side of the item-offset scan's 1024-entry
+    tiles: every record's weighted count and node count, and the item count
+    the run reports, match the oracle and a second split depth."""
+    N = 13
+    rec = C.gen_constellations(N, 5)[:m]
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    for split in (2, 3):
+        total, t = ctx.run_batched(sched(split, 256, 1), count_nodes=True)
+        gw, gn = ctx.results(weighted=True, nodes=True)
+        assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+        assert total == int(w.sum()) and t.items >= 1
