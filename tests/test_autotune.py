@@ -82,3 +82,43 @@ def test_shipped_preset_table_is_usable():
         assert max(4, N - 17) <= p <= 7
         assert solver.preset_for(N) == p
         assert f"{N},{p}" in autotune.TUNED
+
+
+def test_sweep_presets_control_flow(monkeypatch):
+    """sweep_presets with a stand-in context: a preset whose load fails is
+    reported as an error row and skipped, every other preset is swept with
+    the LDS stack only, checked against the expected total, ordered after
+    each load, and its rows carry the preset, record count and load time."""
+    from nqueens import _lib
+
+    class FakeCtx:
+        def __init__(self):
+            self.m, self.loads, self.ordered, self.runs = 0, [], 0, []
+
+        def load(self, N, rec, order=None):
+            self.m = rec.shape[0]
+            self.loads.append(("host", N, self.m))
+
+        def load_expanded(self, N, rec, levels):
+            if levels == 2:
+                raise _lib.NQError(-2, "nq_ctx_load_expanded", "P5 target")
+            self.m = rec.shape[0] * 3
+            self.loads.append(("device", N, levels))
+            return self.m
+
+        def run_batched(self, s):
+            self.runs.append((s.split_depth, s.block, s.lds_stack))
+            return 73712, _lib.Timing(kernel_ms=1.0, total_ms=2.0, items=self.m, launches=1)
+
+    ctx = FakeCtx()
+    order = lambda c: setattr(c, "ordered", c.ordered + 1)
+    rows = autotune.sweep_presets(ctx, 13, 73712, presets=(4, 6, 7), splits=(2, 3), blocks=(64,), reps=1,
+                                  warmup=0, order=order)
+    errs = [r for r in rows if "error" in r]
+    assert [r["preset"] for r in errs] == [7] and "P5 target" in errs[0]["error"]
+    ok = [r for r in rows if "error" not in r]
+    assert [(r["preset"], r["split_depth"]) for r in ok] == [(4, 2), (4, 3), (6, 2), (6, 3)]
+    assert all(r["lds_stack"] == 1 and r["load_ms"] >= 0 and r["records"] > 0 for r in ok)
+    assert ctx.ordered == 2 and all(lds == 1 for _, _, lds in ctx.runs)
+    with pytest.raises(RuntimeError):    # a wrong total is never recorded
+        autotune.sweep_presets(ctx, 13, 1, presets=(4,), splits=(2,), blocks=(64,), reps=1, warmup=0)
