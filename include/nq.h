@@ -137,6 +137,12 @@ int nq_count(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const nq_sche
  * unique = (total + F180 + 2*F90) / 8. */
 int nq_symmetric_counts(nq_ctx *ctx, int N, uint64_t *f90_out, uint64_t *f180_out);
 
+/* ---- small boards ----------------------------------------------------------
+ * Replaces _bit_total (18Py:2071-2084), which main (18Py:2118-2126) uses for
+ * N <= 5 instead of constellations: every solution of the N x N board by a
+ * plain DFS over all rows, on the device.  1 <= N <= 12. */
+int nq_small_total(nq_ctx *ctx, int N, uint64_t *total_out);
+
 /* ---- cost model (no reference counterpart; the reference orders work by a
  * hand-made key, 14Bit_codon_GPU/359Py_pushbranch_partial.py:4342-4404) ----
  * 3-layer FP32 MLP 24 -> 64 -> 64 -> 1 (ReLU), parameters packed as

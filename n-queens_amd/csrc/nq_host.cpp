@@ -544,6 +544,20 @@ int nq_symmetric_counts(nq_ctx *c, int N, uint64_t *f90_out, uint64_t *f180_out)
     return 0;
 }
 
+int nq_small_total(nq_ctx *c, int N, uint64_t *total_out)
+{
+    if (int r = check_ctx(c)) return r;
+    if (N < 1 || N > 12) return nq_fail(NQ_EINVAL, "N=%d outside 1..12", N);
+    if (!total_out) return nq_fail(NQ_EINVAL, "null out pointer");
+    NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->total.p, 1, 0));
+    NQ_HIPCHK(nqk::launch_small_total(c->stream, N, c->total.p));
+    unsigned long long v = 0;
+    NQ_HIPCHK(hipMemcpyAsync(&v, c->total.p, 8, hipMemcpyDeviceToHost, c->stream));
+    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    *total_out = v;
+    return 0;
+}
+
 int nq_decode_host(int N, const uint32_t *rec4, uint64_t m, int32_t *out10)
 {
     if (N < nqt::kMinN || N > nqt::kMaxN) return nq_fail(NQ_EINVAL, "N=%d outside %d..%d", N, nqt::kMinN, nqt::kMaxN);

@@ -49,6 +49,8 @@ hipError_t launch_finalize(hipStream_t s, const uint4 *sched, uint32_t m, const 
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, uint64_t n, unsigned long long v);
 hipError_t launch_fill_u32(hipStream_t s, unsigned int *p, uint64_t n, unsigned int v);
 hipError_t launch_symmetric(hipStream_t s, int N, unsigned long long *out2);
+// *out += every solution of an N x N board, 1 <= N <= 12 (plain DFS)
+hipError_t launch_small_total(hipStream_t s, int N, unsigned long long *out);
 
 // cost model (nq_mlp.hip)
 hipError_t launch_features(hipStream_t s, const uint4 *rec, const uint4 *root, const uint4 *sched,

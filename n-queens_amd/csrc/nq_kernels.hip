@@ -350,6 +350,44 @@ __global__ void k_finalize(const uint4 *__restrict__ sched, uint32_t m, const un
 
 // fill kernel (used instead of hipMemsetAsync so that the sequence can be
 // captured in a graph)
+// Every solution of a small board (N <= kSmallMaxN) by a plain bitboard DFS
+// over all rows: thread c puts the first row's queen in column c.  The
+// reference counts N <= 5 this way instead of through constellations
+// (_bit_total, 18Py:2071-2084).
+constexpr int kSmallMaxN = 12;
+
+__global__ void k_small_total(int N, unsigned long long *__restrict__ out)
+{
+    const int c = threadIdx.x;
+    if (c >= N) return;
+    const uint32_t mask = (1u << N) - 1u;
+    uint32_t sl[kSmallMaxN], sr[kSmallMaxN], sc[kSmallMaxN], sa[kSmallMaxN];
+    uint64_t n = 0;
+    if (N == 1) {
+        n = 1;
+    } else {
+        const uint32_t q = 1u << c;
+        int d = 1;                       // row d: free columns sa[d]; rows 1..N-1
+        sl[1] = q << 1;
+        sr[1] = q >> 1;
+        sc[1] = q;
+        sa[1] = mask & ~(sl[1] | sr[1] | sc[1]);
+        while (d >= 1) {
+            if (sa[d] == 0u) { d--; continue; }
+            const uint32_t bit = sa[d] & (0u - sa[d]);
+            sa[d] ^= bit;
+            if (d == N - 1) { n++; continue; }
+            const uint32_t l = (sl[d] | bit) << 1, r = (sr[d] | bit) >> 1, cc = sc[d] | bit;
+            d++;
+            sl[d] = l;
+            sr[d] = r;
+            sc[d] = cc;
+            sa[d] = mask & ~(l | r | cc);
+        }
+    }
+    if (n) atomicAdd(out, (unsigned long long)n);
+}
+
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, uint64_t n, unsigned long long v)
 {
     for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
@@ -523,6 +561,13 @@ hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32
     k_scan_tops<<<1, 1024, 0, s>>>(tsum, tiles, count);
     if (hipError_t e = hipGetLastError()) return e;
     if (tiles) k_scan_add<<<tiles, kScanTile, 0, s>>>(off, m, tsum);
+    return hipGetLastError();
+}
+
+hipError_t launch_small_total(hipStream_t s, int N, unsigned long long *out)
+{
+    if (N < 1 || N > kSmallMaxN) return hipErrorInvalidValue;
+    k_small_total<<<1, 64, 0, s>>>(N, out);
     return hipGetLastError();
 }
 

@@ -3,7 +3,9 @@
 
 Prints the reference solver's table (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:2115-2160)
 -- N, Total, Unique, hh:mm:ss.ms and ok/ng against A000170 (18Py:2089) --
-with Unique filled in rather than 0.  Every count runs on the GPU.  With
+with Unique filled in rather than 0, from N=5 by default as the reference's
+main (N <= 5 by a plain full-board DFS, 18Py:2118-2126).  Every count runs on
+the GPU.  With
 --chunks K each count runs as K device runs over interleaved slices of the
 constellations and prints a progress line after each; --chunk-range runs
 only some of them (a count split over several runs), --progress-file logs
@@ -29,7 +31,7 @@ def fmt(sec: float) -> str:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m nqueens")
-    ap.add_argument("nmin", type=int, nargs="?", default=6)
+    ap.add_argument("nmin", type=int, nargs="?", default=5)
     ap.add_argument("nmax", type=int, nargs="?", default=None)
     ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
     ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="mlp")

@@ -75,6 +75,7 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_uint64)]),
     "nq_symmetric_counts": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.POINTER(ctypes.c_uint64)]),
+    "nq_small_total": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "nq_ctx_features": (ctypes.c_int, [_vp, _f32p]),
     "nq_mlp_forward": (ctypes.c_int, [_vp, _f32p, _f32p, ctypes.c_uint64, _f32p]),
     "nq_ctx_order_by_cost": (ctypes.c_int, [_vp, _f32p, _vp]),
@@ -239,6 +240,12 @@ class Context:
         check(self.lib.nq_symmetric_counts(self.h, int(N), ctypes.byref(f90), ctypes.byref(f180)),
               "nq_symmetric_counts")
         return int(f90.value), int(f180.value)
+
+    def small_total(self, N: int) -> int:
+        """Every solution of the N x N board by a plain DFS on the device (1 <= N <= 12)."""
+        t = ctypes.c_uint64(0)
+        check(self.lib.nq_small_total(self.h, int(N), ctypes.byref(t)), "nq_small_total")
+        return int(t.value)
 
     def features(self) -> np.ndarray:
         out = np.zeros((self.m, NQ_MLP_IN), np.float32)

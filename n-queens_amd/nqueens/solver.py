@@ -26,7 +26,8 @@ from ._lib import Context, Sched, as_records
 __all__ = ["count", "count_chunks", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
            "preset_for", "constellations_for", "MIN_N", "MAX_N"]
 
-MIN_N, MAX_N = 6, 24
+MIN_N, MAX_N = 6, 24      # the constellation path
+SMALL_N = range(2, 6)     # plain full DFS, as 18Py's main does for N <= 5 (:2118-2126)
 
 _ctx: dict[int, Context] = {}
 _ctx_lock = threading.Lock()
@@ -176,13 +177,20 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
 def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
           sched: Sched | None = None, ctx: Context | None = None, chunks: int = 1,
           progress=None) -> tuple[int, int]:
-    """Total and unique N-Queens solutions for 6 <= N <= 24, counted on the GPU.
+    """Total and unique N-Queens solutions for 2 <= N <= 24, counted on the GPU.
 
-    chunks > 1 counts the constellations in that many slices, one device run
-    each (as the reference counts fixed-size chunks of its list,
-    359Py:5621-5626; see count_chunks), and calls progress(i, chunks,
-    chunk_total) after each: a long count then reports as it goes and no
-    single launch holds the GPU for the whole run."""
+    N = 2..5 are counted by a plain full DFS over the board, as 18Py's main
+    does for N <= 5 (_bit_total, :2071-2084, :2118-2126); N >= 6 through
+    constellations.  chunks > 1 counts the constellations in that many
+    slices, one device run each (as the reference counts fixed-size chunks
+    of its list, 359Py:5621-5626; see count_chunks), and calls
+    progress(i, chunks, chunk_total) after each: a long count then reports
+    as it goes and no single launch holds the GPU for the whole run."""
+    if int(N) in SMALL_N:
+        ctx = ctx or default_context()
+        total = ctx.small_total(int(N))
+        f90, f180 = ctx.symmetric_counts(int(N))
+        return total, unique_count(total, f90, f180)
     _check_n(N)
     if chunks < 1:
         raise ValueError("chunks must be >= 1")

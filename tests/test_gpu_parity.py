@@ -368,3 +368,22 @@ def test_batched_scan_tile_boundaries(ctx, oracle, m):
         gw, gn = ctx.results(weighted=True, nodes=True)
         assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
         assert total == int(w.sum()) and t.items >= 1
+
+
+def test_small_boards(ctx, known, oracle):
+    """N <= 5 by the plain full-board DFS kernel, as the reference's main
+    counts them (_bit_total): totals against A000170 and the brute force,
+    Unique against the published table; N = 6..9 cross-check the kernel
+    against the constellation path."""
+    A000170 = {1: 1, 2: 0, 3: 0, 4: 2, 5: 10}
+    for N, want in A000170.items():
+        assert ctx.small_total(N) == want, N
+    for N in (4, 5):
+        assert oracle.bruteforce(N)[0] == A000170[N]
+    assert nqueens.count(2, ctx=ctx) == (0, 0) and nqueens.count(3, ctx=ctx) == (0, 0)
+    for N in (4, 5):
+        assert list(nqueens.count(N, ctx=ctx)) == known["readme_total_unique"][str(N)]
+    for N in (6, 7, 8, 9):
+        assert ctx.small_total(N) == nqueens.count(N, ctx=ctx)[0] == known["total_a000170"][str(N)]
+    with pytest.raises(nqueens.NQError):
+        ctx.small_total(13)

@@ -122,7 +122,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_solver_argument_checks():
     with pytest.raises(ValueError):
-        nqueens.count(5)
+        nqueens.count(1)      # below the small-board range (2..5) and the constellation range
     with pytest.raises(ValueError):
         nqueens.count(25)
     with pytest.raises(ValueError):
