@@ -154,3 +154,22 @@ def test_bench_refuses_more_than_one_gpu():
     assert res.returncode == 2
     msg = json.loads(res.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+@pytest.mark.parametrize("module,options", [
+    ("nqueens", ["--chunks", "--chunk-range", "--progress-file", "--preset", "--scheduler", "--mode"]),
+    ("nqueens.autotune", ["--presets", "--splits", "--blocks", "--lds", "--chunks", "--reps", "--scheduler"]),
+    ("nqueens.serve", ["--requests", "--nmin", "--nmax", "--load", "--seed", "--scheduler"]),
+    ("nqueens.costmodel", ["--epochs", "--study", "--out"]),
+])
+def test_documented_cli_options_exist(module, options):
+    """The command lines the documentation gives parse (--help exits before any device work)."""
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parent.parent
+    env = dict(__import__("os").environ, PYTHONPATH=f"{root / 'n-queens_amd'}:{root}")
+    res = subprocess.run([sys.executable, "-m", module, "--help"], capture_output=True, text=True, env=env,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr[-2000:]
+    for opt in options:
+        assert opt in res.stdout, (module, opt)
