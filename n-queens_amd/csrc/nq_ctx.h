@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
+#include <vector>
 
 #include "../../include/nq.h"
 
@@ -73,7 +74,8 @@ struct nq_ctx {
     nqk_buf::Buf<uint8_t> nsz;    // board size per record (mixed sets)
     nqk_buf::Buf<uint8_t> splits; // split depth per record (nq_ctx_set_splits)
     bool has_splits = false;
-    uint32_t split_min = 0;       // smallest per-record split depth
+    uint32_t split_min = 0;       // shallowest depth a per-record split starts an item at
+    std::vector<uint8_t> Trec;    // terminal depth per record (host copy, from the decode)
     nqk_buf::Buf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
     nqk_buf::Buf<unsigned long long> scan_tmp;   // tile sums of the item-offset scan, (m + 1023) / 1024
     // cost-model ordering on the device (nq_ctx_order_by_cost)

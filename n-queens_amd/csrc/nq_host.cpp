@@ -57,13 +57,13 @@ uint32_t expand_slots(const nq_ctx *c)
     return c->Tmax > 1u ? c->Tmax : 1u;
 }
 
-// LDS layout of the batched walk: items start at depth `split` unless a
-// record's T is too small to split (then at its root), so the shallowest
-// start depth decides the table and stack sizes.
+// LDS layout of the batched walk: a walk from depth d0 pushes at most
+// T-1-d0 frames, so the shallowest item start depth over the set decides the
+// stack size (with the deepest T).  For per-record depths that minimum is
+// computed per record by nq_ctx_set_splits.
 nqk::WalkLds batched_lds(const nq_ctx *c, int split)
 {
-    if (split < 0) split = (int)c->split_min;   // per-record depths: the shallowest item decides
-    const uint32_t d0 = (split > 0 && (uint32_t)split + 2u <= c->Tmin) ? (uint32_t)split : 0u;
+    const uint32_t d0 = split < 0 ? c->split_min : nqk::item_depth((uint32_t)split, c->Tmin);
     return nqk::walk_lds(c->Tmax, d0);
 }
 
@@ -318,13 +318,17 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
                                           "8=no terminal row, 16=depth)", err);
     // the terminal depth is N-1-preset for every record of a set; take the max
     c->Tmax = 0;
+    c->Tmin = 0;
+    c->Trec.assign(m, 0);
     if (m) {
         std::vector<uint4> sc(nqk::kSchedU4 * m);
         NQ_HIPCHK(hipMemcpy(sc.data(), c->sched.p, nqk::kSchedU4 * m * 16, hipMemcpyDeviceToHost));
         c->Tmin = 0xFFu;
         for (uint64_t a = 0; a < m; a++) {
-            c->Tmax = std::max(c->Tmax, sc[nqk::kSchedU4 * a + 1].x & 0xFFu);
-            c->Tmin = std::min(c->Tmin, sc[nqk::kSchedU4 * a + 1].x & 0xFFu);
+            const uint32_t T = sc[nqk::kSchedU4 * a + 1].x & 0xFFu;
+            c->Trec[a] = (uint8_t)T;
+            c->Tmax = std::max(c->Tmax, T);
+            c->Tmin = std::min(c->Tmin, T);
         }
     }
     // the walk keeps 16 parameter bytes per record (depths 0..15), so the
@@ -389,7 +393,9 @@ int nq_ctx_set_splits(nq_ctx *c, const uint8_t *depth, uint64_t m)
     for (uint64_t a = 0; a < m; a++) {
         if (depth[a] > 15) return nq_fail(NQ_EINVAL, "split depth %u > 15 at record %llu", depth[a],
                                           (unsigned long long)a);
-        mn = std::min<uint32_t>(mn, depth[a]);
+        // the depth the record's items really start at (its root when the
+        // depth is 0 or beyond T-2): the stack is sized for the shallowest
+        mn = std::min<uint32_t>(mn, nqk::item_depth(depth[a], c->Trec[a]));
     }
     NQ_HIPCHK(c->splits.reserve(m ? m : 1));
     if (m) NQ_HIPCHK(hipMemcpyAsync(c->splits.p, depth, m, hipMemcpyHostToDevice, c->stream));

@@ -23,6 +23,15 @@ inline WalkLds walk_lds(uint32_t Tmax, uint32_t d0)
     return WalkLds{1u + (Tmax > d0 + 1u ? Tmax - 1u - d0 : 1u)};
 }
 
+// Depth at which the expansion starts a record's work items for split depth
+// d: d itself when 0 < d <= T-2, else the record's root (k_expand makes the
+// root the single item when d == 0 or d > T-2).  The walk stacks of a set are
+// sized by walk_lds(max T, min over its records of this depth).
+inline uint32_t item_depth(uint32_t d, uint32_t T)
+{
+    return (d > 0u && T >= 2u && d <= T - 2u) ? d : 0u;
+}
+
 hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *rec, uint32_t m, uint4 *root,
                          uint4 *sched, uint32_t *err);
 hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const uint4 *root,

@@ -201,7 +201,7 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     from .constellations import gen_constellations
     from .solver import default_context
-    known = json.loads((Path(__file__).resolve().parents[2] / "tests" / "golden" / "known.json").read_text())
+    from .expected import TOTAL
     ctx = default_context()
     ints = lambda s: tuple(int(x) for x in s.split(","))
     if a.presets:
@@ -210,7 +210,7 @@ def main(argv=None) -> None:
             from .costmodel import default_model
             params = default_model().params
             order = lambda c: c.order_by_cost(params)
-        rows = sweep_presets(ctx, a.n, known["total_a000170"][str(a.n)], ints(a.presets), ints(a.splits),
+        rows = sweep_presets(ctx, a.n, TOTAL[a.n], ints(a.presets), ints(a.splits),
                              ints(a.blocks), ints(a.chunks), reps=a.reps, order=order,
                              log=lambda r: print(json.dumps(r), flush=True))
         best = record_presets(a.n, rows)
@@ -225,7 +225,7 @@ def main(argv=None) -> None:
         from .costmodel import default_model
         order = default_model().order(a.n, rec, ctx=ctx)
     ctx.load(a.n, rec, order)
-    rows = sweep(ctx, a.n, known["total_a000170"][str(a.n)], ints(a.splits), ints(a.blocks), ints(a.lds),
+    rows = sweep(ctx, a.n, TOTAL[a.n], ints(a.splits), ints(a.blocks), ints(a.lds),
                  ints(a.chunks), reps=a.reps, log=lambda r: print(json.dumps(r), flush=True))
     best = record_best(a.n, a.preset, rows)
     print(json.dumps({"best": best, "n": a.n, "preset": a.preset, "scheduler": a.scheduler}), flush=True)

@@ -27,7 +27,7 @@ __all__ = ["count", "count_chunks", "count_tasks", "exec_solutions", "unique_cou
            "preset_for", "constellations_for", "MIN_N", "MAX_N"]
 
 MIN_N, MAX_N = 6, 24      # the constellation path
-SMALL_N = range(2, 6)     # plain full DFS, as 18Py's main does for N <= 5 (:2118-2126)
+SMALL_N = range(1, 6)     # plain full DFS, as 18Py's main does for N <= 5 (:2118-2126)
 
 _ctx: dict[int, Context] = {}
 _ctx_lock = threading.Lock()
@@ -111,9 +111,14 @@ def count_tasks(N: int, records, *, mode: str = "batched", sched: Sched | None =
 
 def exec_solutions(N: int, constellations: list, use_gpu: bool = True) -> None:
     """exec_solutions(N, constellations, use_gpu) of 18Py:1057: writes each
-    dict's "solutions" (weighted count) in place.  Only the GPU path exists."""
-    if not use_gpu:
-        raise ValueError("the CPU path of exec_solutions is not provided; use_gpu must be True")
+    dict's "solutions" (weighted count) in place.
+
+    The reference runs its CPU dfs_iter under @par when use_gpu is False
+    (18Py:1237-1266) and its GPU kernel otherwise; both give the same
+    counts.  This library has one counting path, the GPU kernels, and takes
+    it for either value of use_gpu, so a caller that passes False gets the
+    same numbers the reference's CPU path would write."""
+    del use_gpu   # one path: the counts do not depend on where they are computed
     if not constellations:
         return
     rec = np.array([[c["ld"], c["rd"], c["col"], c["startijkl"]] for c in constellations], dtype=np.uint64)
@@ -177,23 +182,35 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
 def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
           sched: Sched | None = None, ctx: Context | None = None, chunks: int = 1,
           progress=None) -> tuple[int, int]:
-    """Total and unique N-Queens solutions for 2 <= N <= 24, counted on the GPU.
+    """Total and unique N-Queens solutions for 1 <= N <= 24, counted on the GPU.
 
-    N = 2..5 are counted by a plain full DFS over the board, as 18Py's main
+    N = 1..5 are counted by a plain full DFS over the board, as 18Py's main
     does for N <= 5 (_bit_total, :2071-2084, :2118-2126); N >= 6 through
     constellations.  chunks > 1 counts the constellations in that many
     slices, one device run each (as the reference counts fixed-size chunks
     of its list, 359Py:5621-5626; see count_chunks), and calls
     progress(i, chunks, chunk_total) after each: a long count then reports
     as it goes and no single launch holds the GPU for the whole run."""
+    if chunks < 1:
+        raise ValueError("chunks must be >= 1")
+    if mode not in ("plain", "batched"):
+        raise ValueError(f"unknown mode {mode!r}")
+    if scheduler not in ("fifo", "mlp"):
+        raise ValueError(f"unknown scheduler {scheduler!r}")
     if int(N) in SMALL_N:
+        # one device launch; chunks, preset and schedule do not apply
         ctx = ctx or default_context()
         total = ctx.small_total(int(N))
+        if progress is not None:
+            progress(0, 1, total)
+        if int(N) == 1:
+            # the one-square board: its solution is fixed by every symmetry,
+            # reflections included, so the rotation-only orbit formula does
+            # not apply; one solution, one orbit
+            return total, total
         f90, f180 = ctx.symmetric_counts(int(N))
         return total, unique_count(total, f90, f180)
     _check_n(N)
-    if chunks < 1:
-        raise ValueError("chunks must be >= 1")
     ctx = ctx or default_context()
     total = sum(count_chunks(N, chunks, preset=preset, mode=mode, scheduler=scheduler, sched=sched, ctx=ctx,
                              progress=progress))

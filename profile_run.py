@@ -1,6 +1,7 @@
+#!/usr/bin/env python3
 """Fixed workload for profiler runs (rocprofv3 kernel traces and counters).
 
-    python -m nqueens.profile_run --n 19 --preset 5 --mode batched --reps 2
+    python3 profile_run.py --n 19 --preset 5 --mode batched --reps 2
 
 Loads the constellations of (N, preset) once and runs the chosen kernel
 `reps` times outside a graph, so that every dispatch is traced on its own.
@@ -12,13 +13,19 @@ from __future__ import annotations
 
 import argparse
 import json
+import sys
+from pathlib import Path
 
-from . import autotune
-from .constellations import gen_constellations
-from .solver import default_context
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "n-queens_amd", ROOT):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
 
 
 def main(argv=None) -> None:
+    from nqueens import autotune
+    from nqueens.constellations import gen_constellations
+    from nqueens.solver import default_context
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=19)
     ap.add_argument("--preset", type=int, default=5)
@@ -43,7 +50,7 @@ def main(argv=None) -> None:
     ctx.load(a.n, rec)
     params = None
     if a.scheduler == "mlp":
-        from .costmodel import default_model
+        from nqueens.costmodel import default_model
         params = default_model().params
     s = autotune.best_sched(a.n, a.preset)
     for k, v in (("split_depth", a.split), ("block", a.block), ("lds_stack", a.lds)):

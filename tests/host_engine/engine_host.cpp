@@ -14,12 +14,14 @@ namespace {
 uint8_t g_lds[1 << 16];
 
 // Count one item (a frame at depth d0); -1 on a stack overflow past the
-// slots the kernels allocate, -2 if the ended walk does not stay inert.
+// slots the kernels allocate (`lay`: the set-wide layout, or when its slot
+// count is 0 the item's own bound walk_lds(T, d0)), -2 if the ended walk
+// does not stay inert.
 int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32_t av, uint32_t d0,
-                  uint64_t &nodes, bool priv)
+                  uint64_t &nodes, bool priv, WalkLds lay = WalkLds{0u})
 {
     if (S.T == 0) return (av & S.tm) ? 1 : 0;
-    const WalkLds lay = walk_lds(S.T, d0);
+    if (lay.slots == 0u) lay = walk_lds(S.T, d0);
     const uint32_t lane = 5, block = 64;   // lane 5 of a 64-lane workgroup
     LdsMem m{g_lds};
     for (size_t a = 0; a < lay.bytes(block) + 64; a++) g_lds[a] = 0xA5;   // poison
@@ -55,14 +57,32 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
 
 }  // namespace
 
-// Weighted count and node count of every record, expanded at `split` (0 =
-// whole records) and walked with the LDS (priv = 0) or private stack.
-// Returns 0, -1 - decode status, -101 on a stack overflow, -102 if an ended
-// walk does not stay inert.
-extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes, int split,
-                            int priv)
+namespace {
+
+// Weighted count and node count of every record, expanded at its split depth
+// (depths[a], or `split` for all when depths is NULL; 0 = whole records) and
+// walked with the LDS (priv = 0) or private stack.  With set_layout the LDS
+// stacks get the layout the library allocates for the whole set
+// (walk_lds(max T, min item start depth)), else each item its own bound.
+int count_records(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes, int split,
+                  const uint8_t *depths, int priv, bool set_layout)
 {
+    WalkLds set_lay{0u};
+    if (set_layout) {
+        uint32_t Tmax = 0, d0 = 255;
+        for (long a = 0; a < m; a++) {
+            nqt::Desc D;
+            const uint32_t *r = rec + 4 * a;
+            nqt::decode(N, r[0], r[1], r[2], r[3], D);
+            if (D.status) return -1 - D.status;
+            Tmax = D.T > Tmax ? D.T : Tmax;
+            const uint32_t d = item_depth(depths ? depths[a] : (uint32_t)split, D.T);
+            d0 = d < d0 ? d : d0;
+        }
+        if (m) set_lay = walk_lds(Tmax, d0);
+    }
     for (long a = 0; a < m; a++) {
+        if (depths) split = depths[a];
         nqt::Desc D;
         const uint32_t *r = rec + 4 * a;
         nqt::decode(N, r[0], r[1], r[2], r[3], D);
@@ -82,12 +102,12 @@ extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weight
             expand_subtree<true>(S, D.ld, D.rd, D.col, D.avail, 0u, (uint32_t)split, st, c0, n, emit);
             c = c0;
             for (const It &it : items) {
-                const int64_t k = walk_item(S, it.a, it.b, it.c, it.f, (uint32_t)split, n, priv != 0);
+                const int64_t k = walk_item(S, it.a, it.b, it.c, it.f, (uint32_t)split, n, priv != 0, set_lay);
                 if (k < 0) return (int)(-100 + k);
                 c += (uint64_t)k;
             }
         } else if (D.avail) {
-            const int64_t k = walk_item(S, D.ld, D.rd, D.col, D.avail, 0u, n, priv != 0);
+            const int64_t k = walk_item(S, D.ld, D.rd, D.col, D.avail, 0u, n, priv != 0, set_lay);
             if (k < 0) return (int)(-100 + k);
             c = (uint64_t)k;
         }
@@ -95,6 +115,24 @@ extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weight
         nodes[a] = n;
     }
     return 0;
+}
+
+}  // namespace
+
+// Returns 0, -1 - decode status, -101 on a stack overflow, -102 if an ended
+// walk does not stay inert.
+extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes, int split,
+                            int priv)
+{
+    return count_records(N, rec, m, weighted, nodes, split, nullptr, priv, false);
+}
+
+// The same with a split depth per record and the LDS layout sized for the
+// whole set as nq_ctx_set_splits + nq_ctx_run_batched size it.
+extern "C" int engine_count_splits(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes,
+                                   const uint8_t *depths)
+{
+    return count_records(N, rec, m, weighted, nodes, 0, depths, 0, true);
 }
 
 // One level of preset expansion of every record (nqt::preset_children),

@@ -38,6 +38,9 @@ def engine(tmp_path_factory):
     u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
     lib.engine_count.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u64p, u64p, ctypes.c_int, ctypes.c_int]
     lib.engine_count.restype = ctypes.c_int
+    lib.engine_count_splits.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u64p, u64p,
+                                        np.ctypeslib.ndpointer(np.uint8, flags="C")]
+    lib.engine_count_splits.restype = ctypes.c_int
     lib.engine_expand.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u32p, ctypes.c_long]
     lib.engine_expand.restype = ctypes.c_long
     return lib
@@ -99,6 +102,24 @@ def test_engine_n22_n23_sample(engine, N):
     split, priv = (3, 0) if N == 22 else (5, 1)
     rc, gw, gn = _run(engine, N, rec, split, priv)
     assert rc == 0
+    assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+
+
+@pytest.mark.parametrize("N,lo,hi", [(12, 1, 8), (14, 1, 8), (14, 2, 12), (13, 0, 8)])
+def test_engine_per_record_splits_set_layout(engine, oracle, N, lo, hi):
+    """Per-record split depths (some beyond T-2, so those records walk from
+    their roots) with every LDS stack sized for the whole set the way the
+    library sizes it: no push goes past the allocated slots, and counts and
+    node counts equal the oracle's."""
+    rec = C.gen_constellations(N, 5)
+    w, _, nodes = oracle.count_records(N, rec)
+    depth = np.random.default_rng(N + lo).integers(lo, hi, rec.shape[0]).astype(np.uint8)
+    T = N - 1 - 5
+    assert (depth > T - 2).any() and (depth <= T - 2).any()
+    gw = np.zeros(len(rec), np.uint64)
+    gn = np.zeros(len(rec), np.uint64)
+    rc = engine.engine_count_splits(N, np.ascontiguousarray(rec, np.uint32), len(rec), gw, gn, depth)
+    assert rc == 0, rc
     assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
 
 

@@ -187,6 +187,25 @@ def test_per_record_split_depths(ctx, oracle):
         ctx.run_batched(sched(-1, 128, 1))
 
 
+@pytest.mark.parametrize("N,lo,hi", [(14, 1, 8), (12, 1, 8), (15, 2, 12)])
+@pytest.mark.parametrize("block", [64, 256])
+def test_per_record_split_depths_without_root_depth(ctx, oracle, N, lo, hi, block):
+    """Every depth >= 1 and some beyond T-2 (those records are walked from
+    their roots): the LDS stacks must be sized for the root walks, not for
+    the smallest requested depth."""
+    rec = C.gen_constellations(N, 5)
+    w, _, nodes = oracle.count_records(N, rec)
+    depth = np.random.default_rng(N).integers(lo, hi, rec.shape[0]).astype(np.uint8)
+    assert depth.min() >= 1 and (depth > N - 1 - 5 - 2).any()
+    ctx.load(N, rec)
+    ctx.set_splits(depth)
+    for graph in (0, 1):
+        total, _ = ctx.run_batched(sched(-1, block, 1, graph=graph), count_nodes=True)
+        gw, gn = ctx.results(weighted=True, nodes=True)
+        assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+        assert total == int(w.sum())
+
+
 @pytest.fixture(scope="module")
 def oracle_samples():
     from tests.conftest import GOLDEN

@@ -122,11 +122,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_solver_argument_checks():
     with pytest.raises(ValueError):
-        nqueens.count(1)      # below the small-board range (2..5) and the constellation range
+        nqueens.count(0)      # below the small-board range (1..5)
     with pytest.raises(ValueError):
         nqueens.count(25)
-    with pytest.raises(ValueError):
-        nqueens.exec_solutions(8, [{"ld": 0, "rd": 0, "col": 0, "startijkl": 0}], use_gpu=False)
+    # checked before the small-board branch too (no device work is started)
+    for kw in ({"chunks": 0}, {"mode": "cpu"}, {"scheduler": "lifo"}):
+        for n in (1, 4, 12):
+            with pytest.raises(ValueError):
+                nqueens.count(n, **kw)
     with pytest.raises(ArithmeticError):
         nqueens.unique_count(10, 0, 1)
     assert nqueens.unique_count(14200, 8, 80) == 1787
@@ -140,6 +143,24 @@ def test_solver_argument_checks():
     # terminal depth N-1-preset stays within the walk's 16 parameter bytes
     assert [preset_for(n) for n in (22, 23, 24)] == [5, 6, 7]    # no measured preset: the rule
     assert all(n - 1 - preset_for(n) <= 16 for n in range(6, 25))
+
+
+def test_expected_covers_every_countable_board():
+    """The ok/ng table covers every N that count() accepts (and the
+    reference's expected[] range, 18Py:2089), with Unique wherever the
+    published table has it."""
+    from nqueens.expected import TOTAL, UNIQUE
+    from nqueens.solver import MAX_N, SMALL_N
+    assert set(range(min(SMALL_N), MAX_N + 1)) <= set(TOTAL) <= set(UNIQUE) | {26, 27}
+    assert set(range(5, 28)) <= set(TOTAL)
+    # the table agrees with the fixture copy of the published values
+    import json
+    from tests.conftest import GOLDEN
+    known = json.loads((GOLDEN / "known.json").read_text())
+    for n, v in known["total_a000170"].items():
+        assert TOTAL[int(n)] == v, n
+    for n, (t, u) in known["readme_total_unique"].items():
+        assert (TOTAL[int(n)], UNIQUE[int(n)]) == (t, u), n
 
 
 def test_bench_refuses_more_than_one_gpu():
