@@ -129,6 +129,23 @@ int nq_ctx_results(nq_ctx *ctx, uint64_t *weighted_out, uint64_t *nodes_out);
 int nq_count(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const nq_sched *sched,
              uint64_t *weighted_out, uint64_t *nodes_out, uint64_t *total_out);
 
+/* The same count as nq_count, allocating nothing: no hipMalloc, hipFree or
+ * other allocating call.  No counterpart in the reference; for callers that
+ * cannot allocate (code under graph capture, tight serving loops, harnesses
+ * that forbid allocation).  Every device buffer of the call (records,
+ * descriptors, work items, counts) is carved out of one static 128 MiB
+ * device array of the library, which each device holds from its first
+ * context until the process ends; the context only supplies its stream.
+ * Limits: at most 2^18 records (NQ_ECAPACITY beyond), and with a schedule at
+ * most the work items that fit the rest of the array (about 2.2 M; NQ_ECAPACITY
+ * with the count in nq_last_error: use a shallower split depth).  The
+ * schedule's use_graph is ignored (the launches run eagerly) and per-record
+ * split depths (split_depth -1) are refused.  Calls on one device share the
+ * array: calls that overlap on one device race.  The context's resident set
+ * (nq_ctx_load) is left as it was. */
+int nq_count_noalloc(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const nq_sched *sched,
+                     uint64_t *weighted_out, uint64_t *nodes_out, uint64_t *total_out);
+
 /* ---- Unique ---------------------------------------------------------------
  * The constellation solvers print Unique = 0 (18Py:2160).  Unique is the
  * orbit count of 13Bit_codon/07Py_bit_symmetry_codon.py:410-411

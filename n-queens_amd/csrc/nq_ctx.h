@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/nq.h"
+#include "nq_kernels.h"
 
 // Records the message for nq_last_error() and returns `code`.
 int nq_fail(int code, const char *fmt, ...);
@@ -84,6 +85,7 @@ struct nq_ctx {
     nqk_buf::Buf<uint8_t> cm_tmp;
     int sized_split = -2;         // split depth the item buffer is sized for (-1 per-record, -2 none)
     uint64_t item_total = 0;
+    void *scratch = nullptr;      // this device's static scratch array (nq_count_noalloc)
     // captured batched sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -92,6 +94,41 @@ struct nq_ctx {
 
 // Validates the handle and makes its device current.
 int nq_check_ctx(nq_ctx *c);
+
+// Device buffers and shape of one resident set as the runs read them: a
+// context's own buffers, or views into the static scratch array
+// (nq_count_noalloc).
+struct nq_run {
+    uint32_t m = 0, Tmax = 0, Tmin = 0;
+    uint32_t split_min = 0;       // shallowest item start depth of the per-record splits
+    const uint4 *root = nullptr, *sched = nullptr;
+    const uint32_t *order = nullptr;   // processing order, or NULL for record order
+    const uint8_t *splits = nullptr;   // per-record split depths, or NULL
+    uint32_t *nitems = nullptr, *off = nullptr, *head = nullptr;
+    uint4 *items = nullptr;
+    unsigned long long *raw = nullptr, *nodes = nullptr, *weighted = nullptr, *total = nullptr;
+    unsigned long long *item_count = nullptr, *scan_tmp = nullptr;
+};
+
+// Checks and completes a batched schedule for the set (block size, claim
+// chunk, split depth clamped to T-2, LDS stack within lds_max).
+int nq_prepare_sched(const nq_run &r, int lds_max, nq_sched *s);
+// LDS layout of the batched walk for split depth `split` (-1: per record).
+nqk::WalkLds nq_batched_lds(const nq_run &r, int split);
+// Persistent grid size (occupancy, waves_per_eu cap) and the u32 queue-head bound.
+int nq_batched_grid(int cu, const nq_sched &s, size_t lds_bytes, int nodes, uint64_t items, int *grid);
+// The batched launch sequence on stream st: clear, expand (count, scan,
+// write), persistent DFS, finalize.  ev (4 events, may be NULL) brackets the
+// sequence (0, 3) and the DFS kernel (1, 2).
+int nq_enqueue_batched(hipStream_t st, const nq_run &r, const nq_sched &s, int nodes, int grid,
+                       hipEvent_t *ev);
+// Expansion count pass + scan of the set at split depth `split`, with the item
+// total read back into *count.
+int nq_count_items(hipStream_t st, const nq_run &r, int split, int block, uint64_t *count);
+
+// The static scratch array of this device (nq_noalloc.hip): its address,
+// looked up once per device by nq_ctx_create.
+int nq_scratch_attach(int device, void **ptr);
 
 #define NQ_CTX_CHECK(c)                       \
     do {                                      \

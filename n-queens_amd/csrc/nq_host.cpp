@@ -51,20 +51,35 @@ void drop_graph(nq_ctx *c)
     c->gkey = nq_graph_key{};
 }
 
-// Stack slots per lane of the expansion DFS (frames of depths 0..T-2).
-uint32_t expand_slots(const nq_ctx *c)
+// The resident set of a context as the runs read it.
+nq_run ctx_run(nq_ctx *c)
 {
-    return c->Tmax > 1u ? c->Tmax : 1u;
+    nq_run r;
+    r.m = (uint32_t)c->m;
+    r.Tmax = c->Tmax;
+    r.Tmin = c->Tmin;
+    r.split_min = c->split_min;
+    r.root = c->root.p;
+    r.sched = c->sched.p;
+    r.order = c->order.p;
+    r.splits = c->has_splits ? c->splits.p : nullptr;
+    r.nitems = c->nitems.p;
+    r.off = c->off.p;
+    r.head = c->head.p;
+    r.items = c->items.p;
+    r.raw = c->raw.p;
+    r.nodes = c->nodes.p;
+    r.weighted = c->weighted.p;
+    r.total = c->total.p;
+    r.item_count = c->item_count.p;
+    r.scan_tmp = c->scan_tmp.p;
+    return r;
 }
 
-// LDS layout of the batched walk: a walk from depth d0 pushes at most
-// T-1-d0 frames, so the shallowest item start depth over the set decides the
-// stack size (with the deepest T).  For per-record depths that minimum is
-// computed per record by nq_ctx_set_splits.
-nqk::WalkLds batched_lds(const nq_ctx *c, int split)
+// Stack slots per lane of the expansion DFS (frames of depths 0..T-2).
+uint32_t expand_slots(const nq_run &r)
 {
-    const uint32_t d0 = split < 0 ? c->split_min : nqk::item_depth((uint32_t)split, c->Tmin);
-    return nqk::walk_lds(c->Tmax, d0);
+    return r.Tmax > 1u ? r.Tmax : 1u;
 }
 
 int elapsed(nq_ctx *c, int a, int b, double *ms)
@@ -75,18 +90,11 @@ int elapsed(nq_ctx *c, int a, int b, double *ms)
     return 0;
 }
 
-// Item count of the split at `split` (expansion count pass + scan, read back).
+// Item count of the split at `split`; sizes the context's item buffer.
 int size_items(nq_ctx *c, int split, int block)
 {
-    const uint32_t sd = expand_slots(c);
-    NQ_HIPCHK(nqk::launch_expand(c->stream, false, false, block, sd, c->root.p, c->sched.p,
-                              c->order.p, split < 0 ? c->splits.p : nullptr, (uint32_t)c->m,
-                              (uint32_t)(split < 0 ? 0 : split), c->nitems.p, nullptr, nullptr,
-                              nullptr, nullptr));
-    NQ_HIPCHK(nqk::launch_scan(c->stream, c->nitems.p, (uint32_t)c->m, c->off.p, c->item_count.p, c->scan_tmp.p));
-    unsigned long long cnt = 0;
-    NQ_HIPCHK(hipMemcpyAsync(&cnt, c->item_count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    NQ_HIPCHK(hipStreamSynchronize(c->stream));
+    uint64_t cnt = 0;
+    if (int r = nq_count_items(c->stream, ctx_run(c), split, block, &cnt)) return r;
     // item indices and offsets are u32 (the queue head's claims past the end
     // are checked against the launch in nq_ctx_run_batched)
     if (cnt >= (1ull << 32)) return nq_fail(NQ_ECAPACITY, "split depth %d gives %llu work items (limit 2^32)", split, cnt);
@@ -96,39 +104,90 @@ int size_items(nq_ctx *c, int split, int block)
     return 0;
 }
 
-// The batched launch sequence: clear, expand (count, scan, write), persistent
-// DFS, finalize: every run recomputes the whole expansion from the resident
-// records (size_items has only sized the item buffer).  Events 0/3 bracket
-// the sequence, 1/2 the DFS kernel.
-int enqueue_batched(nq_ctx *c, const nq_sched &s, int nodes, int grid, bool events)
+}  // namespace
+
+// LDS layout of the batched walk: a walk from depth d0 pushes at most
+// T-1-d0 frames, so the shallowest item start depth over the set decides the
+// stack size (with the deepest T).  For per-record depths that minimum is
+// computed per record by nq_ctx_set_splits.
+nqk::WalkLds nq_batched_lds(const nq_run &r, int split)
 {
-    const uint32_t m = (uint32_t)c->m, sd = expand_slots(c);
-    const nqk::WalkLds lay = batched_lds(c, s.split_depth);
-    hipStream_t st = c->stream;
-    if (events) NQ_HIPCHK(hipEventRecord(c->ev[0], st));
-    NQ_HIPCHK(nqk::launch_fill_u64(st, c->raw.p, m, 0));
-    NQ_HIPCHK(nqk::launch_fill_u64(st, c->total.p, 1, 0));
-    NQ_HIPCHK(nqk::launch_fill_u32(st, c->head.p, 1, 0));
-    if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, c->nodes.p, m, 0));
-    const uint8_t *sp8 = s.split_depth < 0 ? c->splits.p : nullptr;
-    const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
-    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
-                              su, c->nitems.p, nullptr, nullptr, nullptr, nullptr));
-    NQ_HIPCHK(nqk::launch_scan(st, c->nitems.p, m, c->off.p, c->item_count.p, c->scan_tmp.p));
-    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, c->root.p, c->sched.p, c->order.p, sp8, m,
-                              su, c->nitems.p, c->off.p, c->items.p,
-                              c->raw.p, c->nodes.p));
-    if (events) NQ_HIPCHK(hipEventRecord(c->ev[1], st));
-    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, lay, c->items.p,
-                               c->item_count.p, c->head.p, (uint32_t)s.chunk,
-                               c->raw.p, c->nodes.p));
-    if (events) NQ_HIPCHK(hipEventRecord(c->ev[2], st));
-    NQ_HIPCHK(nqk::launch_finalize(st, c->sched.p, m, c->raw.p, c->weighted.p, c->total.p));
-    if (events) NQ_HIPCHK(hipEventRecord(c->ev[3], st));
+    const uint32_t d0 = split < 0 ? r.split_min : nqk::item_depth((uint32_t)split, r.Tmin);
+    return nqk::walk_lds(r.Tmax, d0);
+}
+
+int nq_prepare_sched(const nq_run &r, int lds_max, nq_sched *s)
+{
+    if (s->block != 64 && s->block != 128 && s->block != 256) return nq_fail(NQ_EINVAL, "block must be 64, 128 or 256");
+    if (s->chunk < 1) s->chunk = 64;
+    if (s->split_depth < -1) return nq_fail(NQ_EINVAL, "split depth must be >= 0, or -1 for per-record depths");
+    if (s->split_depth == -1 && !r.splits) return nq_fail(NQ_ESTATE, "no per-record split depths set");
+    // items live at depth <= T-2 (a deeper split falls back to the root item)
+    const int max_split = r.Tmax >= 2 ? (int)r.Tmax - 2 : 0;
+    if (s->split_depth > max_split) s->split_depth = max_split;
+    const nqk::WalkLds lay = nq_batched_lds(r, s->split_depth);
+    const size_t lds_bytes = s->lds_stack ? lay.bytes(s->block) : 0u;
+    if (lds_bytes > (size_t)lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, lds_max);
     return 0;
 }
 
-}  // namespace
+int nq_batched_grid(int cu, const nq_sched &s, size_t lds_bytes, int nodes, uint64_t items, int *grid)
+{
+    int per_cu = 0;
+    NQ_HIPCHK(nqk::batched_occupancy(nodes != 0, s.lds_stack != 0, s.block, lds_bytes, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    if (s.waves_per_eu > 0) {
+        int want = (s.waves_per_eu * 4 * 64) / s.block;   // 4 SIMDs per CU
+        if (want < 1) want = 1;
+        per_cu = std::min(per_cu, want);
+    }
+    *grid = cu * per_cu;
+    // the u32 queue head ends at most one chunk per wave past the item count
+    if (items + (uint64_t)*grid * (s.block / 64) * (uint64_t)s.chunk >= (1ull << 32))
+        return nq_fail(NQ_ECAPACITY, "%llu work items and %d claims of %d past the end overflow the u32 queue head",
+                       (unsigned long long)items, *grid * (s.block / 64), s.chunk);
+    return 0;
+}
+
+int nq_count_items(hipStream_t st, const nq_run &r, int split, int block, uint64_t *count)
+{
+    NQ_HIPCHK(nqk::launch_expand(st, false, false, block, expand_slots(r), r.root, r.sched, r.order,
+                                 split < 0 ? r.splits : nullptr, r.m, (uint32_t)(split < 0 ? 0 : split), r.nitems,
+                                 nullptr, nullptr, nullptr, nullptr));
+    NQ_HIPCHK(nqk::launch_scan(st, r.nitems, r.m, r.off, r.item_count, r.scan_tmp));
+    unsigned long long cnt = 0;
+    NQ_HIPCHK(hipMemcpyAsync(&cnt, r.item_count, sizeof cnt, hipMemcpyDeviceToHost, st));
+    NQ_HIPCHK(hipStreamSynchronize(st));
+    *count = cnt;
+    return 0;
+}
+
+// Every run recomputes the whole expansion from the resident records (the
+// sizing pass has only sized the item buffer).
+int nq_enqueue_batched(hipStream_t st, const nq_run &r, const nq_sched &s, int nodes, int grid, hipEvent_t *ev)
+{
+    const uint32_t m = r.m, sd = expand_slots(r);
+    const nqk::WalkLds lay = nq_batched_lds(r, s.split_depth);
+    if (ev) NQ_HIPCHK(hipEventRecord(ev[0], st));
+    NQ_HIPCHK(nqk::launch_fill_u64(st, r.raw, m, 0));
+    NQ_HIPCHK(nqk::launch_fill_u64(st, r.total, 1, 0));
+    NQ_HIPCHK(nqk::launch_fill_u32(st, r.head, 1, 0));
+    if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, r.nodes, m, 0));
+    const uint8_t *sp8 = s.split_depth < 0 ? r.splits : nullptr;
+    const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
+    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, r.root, r.sched, r.order, sp8, m,
+                                 su, r.nitems, nullptr, nullptr, nullptr, nullptr));
+    NQ_HIPCHK(nqk::launch_scan(st, r.nitems, m, r.off, r.item_count, r.scan_tmp));
+    NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, r.root, r.sched, r.order, sp8, m,
+                                 su, r.nitems, r.off, r.items, r.raw, r.nodes));
+    if (ev) NQ_HIPCHK(hipEventRecord(ev[1], st));
+    NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, lay, r.items,
+                                  r.item_count, r.head, (uint32_t)s.chunk, r.raw, r.nodes));
+    if (ev) NQ_HIPCHK(hipEventRecord(ev[2], st));
+    NQ_HIPCHK(nqk::launch_finalize(st, r.sched, m, r.raw, r.weighted, r.total));
+    if (ev) NQ_HIPCHK(hipEventRecord(ev[3], st));
+    return 0;
+}
 
 extern "C" {
 
@@ -156,6 +215,8 @@ int init_ctx(nq_ctx *c)
     NQ_HIPCHK(c->head.reserve(1));
     NQ_HIPCHK(c->item_count.reserve(1));
     NQ_HIPCHK(c->sym.reserve(2));
+    // the device address of this device's static scratch array (nq_count_noalloc)
+    if (int r = nq_scratch_attach(c->device, &c->scratch)) return r;
     return 0;
 }
 
@@ -442,16 +503,9 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     if (!c->loaded) return nq_fail(NQ_ESTATE, "no task set loaded");
     if (!sp) return nq_fail(NQ_EINVAL, "null schedule");
     nq_sched s = *sp;
-    if (s.block != 64 && s.block != 128 && s.block != 256) return nq_fail(NQ_EINVAL, "block must be 64, 128 or 256");
-    if (s.chunk < 1) s.chunk = 64;
-    if (s.split_depth < -1) return nq_fail(NQ_EINVAL, "split depth must be >= 0, or -1 for per-record depths");
+    nq_run run = ctx_run(c);
     if (s.split_depth == -1 && !c->has_splits) return nq_fail(NQ_ESTATE, "no per-record split depths set");
-    // items live at depth <= T-2 (a deeper split falls back to the root item)
-    const int max_split = c->Tmax >= 2 ? (int)c->Tmax - 2 : 0;
-    if (s.split_depth > max_split) s.split_depth = max_split;
-    const nqk::WalkLds lay = batched_lds(c, s.split_depth);
-    const size_t lds_bytes = s.lds_stack ? lay.bytes(s.block) : 0u;
-    if (lds_bytes > (size_t)c->lds_max) return nq_fail(NQ_ECAPACITY, "LDS stack %zu B > %d B", lds_bytes, c->lds_max);
+    if (int r = nq_prepare_sched(run, c->lds_max, &s)) return r;
     if (c->m == 0) {
         if (total_out) *total_out = 0;
         if (t) memset(t, 0, sizeof *t);
@@ -460,20 +514,11 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
     if (c->sized_split != s.split_depth) {
         drop_graph(c);
         if (int r = size_items(c, s.split_depth, s.block)) return r;
+        run.items = c->items.p;
     }
-    int per_cu = 0;
-    NQ_HIPCHK(nqk::batched_occupancy(count_nodes != 0, s.lds_stack != 0, s.block, lds_bytes, &per_cu));
-    if (per_cu < 1) per_cu = 1;
-    if (s.waves_per_eu > 0) {
-        int want = (s.waves_per_eu * 4 * 64) / s.block;   // 4 SIMDs per CU
-        if (want < 1) want = 1;
-        per_cu = std::min(per_cu, want);
-    }
-    const int grid = c->cu * per_cu;
-    // the u32 queue head ends at most one chunk per wave past the item count
-    if (c->item_total + (uint64_t)grid * (s.block / 64) * (uint64_t)s.chunk >= (1ull << 32))
-        return nq_fail(NQ_ECAPACITY, "%llu work items and %d claims of %d past the end overflow the u32 queue head",
-                       (unsigned long long)c->item_total, grid * (s.block / 64), s.chunk);
+    const size_t lds_bytes = s.lds_stack ? nq_batched_lds(run, s.split_depth).bytes(s.block) : 0u;
+    int grid = 0;
+    if (int r = nq_batched_grid(c->cu, s, lds_bytes, count_nodes, c->item_total, &grid)) return r;
     if (s.use_graph) {
         nq_graph_key k;
         k.split = s.split_depth; k.block = s.block; k.lds = s.lds_stack; k.chunk = s.chunk; k.grid = grid;
@@ -481,7 +526,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         if (!c->gexec || !(k == c->gkey)) {
             drop_graph(c);
             NQ_HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int r = enqueue_batched(c, s, count_nodes, grid, false);
+            int r = nq_enqueue_batched(c->stream, run, s, count_nodes, grid, nullptr);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (r) { if (g) (void)hipGraphDestroy(g); return r; }
@@ -495,7 +540,7 @@ int nq_ctx_run_batched(nq_ctx *c, const nq_sched *sp, int count_nodes, uint64_t 
         NQ_HIPCHK(hipGraphLaunch(c->gexec, c->stream));
         NQ_HIPCHK(hipEventRecord(c->ev[3], c->stream));
     } else {
-        if (int r = enqueue_batched(c, s, count_nodes, grid, true)) return r;
+        if (int r = nq_enqueue_batched(c->stream, run, s, count_nodes, grid, c->ev)) return r;
     }
     unsigned long long tot = 0, items = 0;
     NQ_HIPCHK(hipMemcpyAsync(&tot, c->total.p, 8, hipMemcpyDeviceToHost, c->stream));

@@ -73,6 +73,8 @@ SIGNATURES = {
     "nq_ctx_results": (ctypes.c_int, [_vp, _vp, _vp]),
     "nq_count": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(Sched), _vp, _vp,
                                 ctypes.POINTER(ctypes.c_uint64)]),
+    "nq_count_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(Sched), _vp, _vp,
+                                        ctypes.POINTER(ctypes.c_uint64)]),
     "nq_symmetric_counts": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.POINTER(ctypes.c_uint64)]),
     "nq_small_total": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
