@@ -179,6 +179,7 @@ typedef struct {
     uint64_t LD, RD, mask;
     sset *cache;      /* 18Py keeps one cache for the whole call (cleared at :1681) */
     sset *sigs;       /* per-sc signature set, preset <= 5 (:1595-1604) */
+    int nosig;        /* record expansion: every leaf is kept */
     uint64_t *rec;    /* 4 x u64 per record: ld, rd, col, startijkl */
     size_t nrec, caprec;
     long cache_hits, cache_hits_cross_sc, sig_hits;
@@ -224,7 +225,7 @@ static void set_pre_queens(genctx *g, uint64_t ld, uint64_t rd, uint64_t col, in
         return;
     }
     if (queens == g->preset) {                             /* :1594 leaf */
-        if (g->preset <= 5) {                              /* :1596 signature dedup */
+        if (g->preset <= 5 && !g->nosig) {                 /* :1596 signature dedup */
             skey sig;
             sig.ld = ld; sig.rd = rd; sig.col = col; sig.LD = 0; sig.RD = 0;
             sig.small = (uint64_t)g->k | ((uint64_t)g->l << 8) | ((uint64_t)row << 16);
@@ -327,6 +328,45 @@ NQO_EXPORT long nqo_gen_constellations_scoped(int N, int preset, int scope, uint
     return rc;
 }
 
+/*
+ * nqo_expand_records: the records of preset p + levels that set_pre_queens
+ * (18Py:1535-1613) reaches from each given record of preset p, continuing
+ * its recursion from the record's state (start row, masks) with the
+ * diagonals LD/RD of its border queens (:1749-1750) and no cache or
+ * signature dedup; records in input order, columns ascending within one.
+ * Writes at most cap records to out (u32 {ld, rd, col, startijkl}); ld
+ * bits at 32 and above are dropped (after build_soa's ld >> 1 they lie at
+ * bit 31 or above, off every board up to N = 31, and ld only shifts up).
+ * Returns the record count, -1 when another field does not fit 32 bits, -2
+ * when cap is too small.
+ */
+NQO_EXPORT long nqo_expand_records(int N, const uint32_t *rec, long m, int levels, uint32_t *out, long cap)
+{
+    genctx g; memset(&g, 0, sizeof g);
+    g.N = N; g.preset = levels; g.scope = 2; g.nosig = 1;
+    g.mask = (1ULL << N) - 1;
+    const uint64_t L = 1ULL << (N - 1);
+    for (long a = 0; a < m; a++) {
+        const uint32_t *r = rec + 4 * a;
+        const int ijkl = (int)(r[3] & 0xFFFFF), row = (int)(r[3] >> 20);
+        const int j = getj(ijkl), k = getk(ijkl), l = getl(ijkl);
+        g.LD = (L >> j) | (L >> l);
+        g.RD = (L >> j) | (1ULL << k);
+        g.k = k; g.l = l;
+        size_t before = g.nrec;
+        set_pre_queens(&g, r[0], r[1], r[2], row, 0);
+        for (size_t b = before; b < g.nrec; b++) g.rec[4 * b + 3] |= (uint64_t)ijkl;
+    }
+    long rc = (long)g.nrec;
+    if (rc > cap) rc = -2;
+    for (size_t b = 0; rc >= 0 && b < 4 * g.nrec; b++) {
+        if ((b & 3) != 0 && (g.rec[b] >> 32)) rc = -1;
+        else out[b] = (uint32_t)g.rec[b];
+    }
+    free(g.rec);
+    return rc;
+}
+
 NQO_EXPORT long nqo_gen_constellations(int N, int preset, uint32_t **out, long *stats)
 {
     return nqo_gen_constellations_scoped(N, preset, 0, out, stats);
@@ -359,6 +399,16 @@ typedef struct {
     int start, fid, jmark, endmark, mark1, mark2, w, ijkl;
 } task_t;
 
+/* Width of the rd word build_soa's gate assumes: the j queen's rd bit is set
+ * at the start row when its position (N-1-j) + (N-2-start) is below it
+ * (18Py:814-815, j >= 2N-33-start for 32 bits), and the target tree sends
+ * the other records to the P5 functions (j > 2N-34-start, 18Py:849), which
+ * set it when it reaches the board.  32 is the reference; the tests lower it
+ * (any width >= N-1 must give the same counts) to put P5 records on small
+ * boards. */
+static int g_gate_bits = 32;
+NQO_EXPORT void nqo_set_gate_bits(int w) { g_gate_bits = w; }
+
 static int build_task(int N, const uint32_t *rec, task_t *t)
 {
     const uint64_t board_mask = (1ULL << N) - 1;
@@ -380,7 +430,9 @@ static int build_task(int N, const uint32_t *rec, task_t *t)
         if (sh < 0) return -1;
         rd |= 1ULL << sh;
     }
-    if (j >= 2 * N - 33 - start) {
+    /* (N1-j) + (N2-start) <= gate_bits - 1, i.e. j >= 2N-33-start at 32 bits */
+    const int j_gate = j >= 2 * N - 1 - g_gate_bits - start;
+    if (j_gate) {
         if (N2 - start < 0) return -1;
         rd |= (1ULL << (N1 - j)) << (N2 - start);
     }
@@ -388,7 +440,6 @@ static int build_task(int N, const uint32_t *rec, task_t *t)
     int j_lt_N3 = j < N - 3, j_eq_N3 = j == N - 3, j_eq_N2 = j == N - 2;
     int k_lt_l = k < l, start_lt_k = start < k, start_lt_l = start < l;
     int l_eq_kp1 = l == k + 1, k_eq_lp1 = k == l + 1;
-    int j_gate = j > 2 * N - 34 - start;
     if (j_lt_N3) {                                         /* 18Py:839-906 */
         jmark = j + 1; endmark = N2;
         if (j_gate) {
@@ -480,8 +531,19 @@ NQO_EXPORT int nqo_build_soa(int N, const uint32_t *rec, long m, int32_t *out8)
 typedef struct { uint64_t ld, rd, col, fr; int fid, row; } frame_t;
 #define NQO_STACK 2048
 
-static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
+/* prune = 0: the func_meta availptn table of 18Py (:1103-1132), the node
+ * definition of the product.  prune = 1: the pruning of the reference's
+ * recursive SQ* functions (13Bit_codon/13Py_constellations_codon.py:183-678),
+ * which differs in three places only: SQBjrB (fid 2) and the SQBjl* (fids
+ * 8-11) place without the one-row lookahead, and SQd1BlB (fid 20) applies it
+ * on its mark2 step (:306-314).  Counts are the same (every lookahead is a
+ * sound prune); node counts differ. */
+#define NQO_PRUNE_13PY 1
+#define NQO_PLANT_BASE 2     /* planted mistake (tests only): a base frame counts as a node */
+#define NQO_PLANT_JMARK 4    /* planted mistake (tests only): column 0 counted before the jmark mask */
+static int dfs_iter_mode(int N, const task_t *t, uint64_t *count, uint64_t *nodes, int mode)
 {
+    const int prune = mode & NQO_PRUNE_13PY;
     const uint64_t board_mask = (1ULL << N) - 1;
     const uint64_t n3 = 1ULL << (N > 3 ? N - 3 : 0), n4 = 1ULL << (N > 4 ? N - 4 : 0);
     frame_t st[NQO_STACK];
@@ -496,10 +558,11 @@ static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
         if (ptn == 5 && f.row == t->endmark) {             /* base */
             if (fid == 14) total += (avail >> 1) ? 1 : 0;
             else total += 1;
+            if (mode & NQO_PLANT_BASE) nn++;
             continue;
         }
         int step = 1, add1 = 0, row_step = f.row + 1, use_blocks = 0;
-        int use_future = META_AVAIL[fid] == 1, lnext = fid;
+        int use_future = META_AVAIL[fid] == 1 && !(prune == 1 && fid == 2), lnext = fid;
         uint64_t bK = 0, bL = 0;
         if (ptn == 0 || ptn == 1 || ptn == 2) {             /* mark rows */
             int at_mark = (ptn == 0 || ptn == 2) ? (f.row == t->mark1) : (f.row == t->mark2);
@@ -512,14 +575,29 @@ static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
                 use_blocks = 1; use_future = 0; lnext = nxt;
             }
         } else if (ptn == 3 && f.row == t->jmark) {         /* jmark row */
+            if ((mode & NQO_PLANT_JMARK) && (avail & 1)) nn++;
             avail &= ~1ULL;
             ld |= 1;
             lnext = nxt;
             if (!avail) continue;
         } else if (ptn == 4) {
-            return -2;   /* P5 targets (fids 8-11): none for N <= 23 at presets 5-6 */
+            /* P5 (fids 8-11, "SQBjl*"), 13Bit_codon/13Py_constellations_codon.py:620-678
+             * (= 11Bit_Python/17Py_constellations.py:406-465): these records start above
+             * the row where the j queen's down-right diagonal enters the board, the bit
+             * build_soa leaves out (its 32-bit gate, 18Py:814-815).  At row N-1-jmark
+             * the frame takes that bit (rd |= 1 << (N-1), column N-1 no longer free) and
+             * continues as next_funcid on the same row; rows above it are placed without
+             * the one-row lookahead, as the SQBjl loops do. */
+            if (f.row == N - 1 - t->jmark) {
+                const uint64_t bj = 1ULL << (N - 1);
+                if (sp >= NQO_STACK) return -1;
+                st[sp++] = (frame_t){ld, f.rd | bj, f.col, avail & ~bj, nxt, f.row};
+                continue;
+            }
+            use_future = 0;
         }
         if (use_blocks) {
+            const int mlook = prune == 1 && fid == 20 && row_step < t->endmark;
             while (avail) {
                 uint64_t bit = avail & (0 - avail); avail &= avail - 1; nn++;
                 uint64_t nld = ((ld | bit) << step) | (uint64_t)add1 | bL;
@@ -527,6 +605,7 @@ static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
                 uint64_t ncol = f.col | bit;
                 uint64_t nf = board_mask & ~(nld | nrd | ncol);
                 if (nf) {
+                    if (mlook && !(board_mask & ~((nld << 1) | (nrd >> 1) | ncol))) continue;
                     if (sp >= NQO_STACK) return -1;
                     st[sp++] = (frame_t){nld, nrd, ncol, nf, lnext, row_step};
                 }
@@ -547,6 +626,11 @@ static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
     *count = total;
     if (nodes) *nodes = nn;
     return 0;
+}
+
+static int dfs_iter(int N, const task_t *t, uint64_t *count, uint64_t *nodes)
+{
+    return dfs_iter_mode(N, t, count, nodes, 0);
 }
 
 /* Recursive form of the same count, 18Py:564-696 (`dfs`).  Kept as an
@@ -577,6 +661,13 @@ static uint64_t dfs_rec(int N, const task_t *t, int fid, uint64_t ld, uint64_t r
     } else if (ptn == 3 && row == t->jmark) {
         avail &= ~1ULL; ld |= 1; lnext = nxt;
         if (!avail) return 0;
+    } else if (ptn == 4) {
+        /* P5, 13Py:620-678: the j diagonal enters at row N-1-jmark */
+        if (row == N - 1 - t->jmark) {
+            const uint64_t bj = 1ULL << (N - 1);
+            return dfs_rec(N, t, nxt, ld, rd | bj, col, row, avail & ~bj, nodes);
+        }
+        use_future = 0;
     }
     while (avail) {
         uint64_t bit = avail & (0 - avail);
@@ -606,7 +697,7 @@ static uint64_t dfs_rec(int N, const task_t *t, int fid, uint64_t ld, uint64_t r
  * (A6), weighted count = count * w (A3, 18Py:1266).
  * weighted/raw/nodes may be NULL.  threads <= 0 uses the OpenMP default.
  * schedule: 0 = static (Codon @par), 1 = dynamic,64.
- * Returns 0, -1 (stack capacity), -2 (P5 target), -3 (shift out of range).
+ * Returns 0, -1 (stack capacity), -3 (shift out of range).
  */
 NQO_EXPORT int nqo_count_records(int N, const uint32_t *rec, long m, uint64_t *weighted,
                                  uint64_t *raw, uint64_t *nodes, int threads, int schedule)
@@ -636,6 +727,29 @@ NQO_EXPORT int nqo_count_records(int N, const uint32_t *rec, long m, uint64_t *w
         for (long a = 0; a < m; a++) NQO_BODY
     }
 #undef NQO_BODY
+    return err;
+}
+
+/* Weighted counts and node counts under the pruning of the reference's
+ * recursive SQ* functions (dfs_iter_mode with NQO_PRUNE_13PY).  plant: 0, or
+ * NQO_PLANT_BASE / NQO_PLANT_JMARK to plant a node-accounting mistake (the
+ * tests show that the reference's node census catches it). */
+NQO_EXPORT int nqo_count_records_13py(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes,
+                                      int plant)
+{
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long a = 0; a < m; a++) {
+        task_t t;
+        uint64_t c = 0, nn = 0;
+        int rc = build_task(N, rec + 4 * a, &t) ? -3 : dfs_iter_mode(N, &t, &c, &nn, NQO_PRUNE_13PY | plant);
+        if (rc) {
+#pragma omp atomic write
+            err = rc;
+        }
+        weighted[a] = c * (uint64_t)t.w;
+        nodes[a] = nn;
+    }
     return err;
 }
 

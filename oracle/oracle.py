@@ -50,9 +50,13 @@ def lib() -> ctypes.CDLL:
         L.nqo_count_records.argtypes = [ctypes.c_int, _u32p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.nqo_count_records_recursive.argtypes = [ctypes.c_int, _u32p, ctypes.c_long, _u64p, _u64p]
+        L.nqo_count_records_13py.argtypes = [ctypes.c_int, _u32p, ctypes.c_long, _u64p, _u64p, ctypes.c_int]
         L.nqo_build_soa.argtypes = [ctypes.c_int, _u32p, ctypes.c_long, _i32p]
         L.nqo_symmetric_counts.argtypes = [ctypes.c_int, _u64p, _u64p]
         L.nqo_bruteforce.argtypes = [ctypes.c_int, _u64p]
+        L.nqo_expand_records.restype = ctypes.c_long
+        L.nqo_expand_records.argtypes = [ctypes.c_int, _u32p, ctypes.c_long, ctypes.c_int, _u32p, ctypes.c_long]
+        L.nqo_set_gate_bits.argtypes = [ctypes.c_int]
         L.nqo_jasmin.argtypes = [ctypes.c_int, ctypes.c_int]
         L.nqo_symmetry.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -74,6 +78,18 @@ def gen_constellations(N: int, preset: int = 5, scope: int = SCOPE_NONE):
     a = np.ctypeslib.as_array(p, shape=(max(n, 1) * 4,)).copy().reshape(-1, 4)[:n]
     L.nqo_free(p)
     return np.ascontiguousarray(a), list(st)
+
+
+def expand_records(N: int, rec, levels: int = 1) -> np.ndarray:
+    """Records of preset p + levels reached from each record of preset p
+    (set_pre_queens continued from the record; input order, columns ascending)."""
+    rec = np.ascontiguousarray(np.asarray(rec, dtype=np.uint32))
+    cap = max(1, rec.shape[0]) * N ** levels
+    out = np.zeros(cap * 4, np.uint32)
+    n = lib().nqo_expand_records(N, rec, rec.shape[0], levels, out, cap)
+    if n < 0:
+        raise OverflowError(f"oracle expand_records failed ({n})")
+    return out[: 4 * n].reshape(-1, 4).copy()
 
 
 def count_records(N: int, rec, threads: int = 0, schedule: int = 1, nodes: bool = True):
@@ -99,6 +115,39 @@ def count_records_recursive(N: int, rec):
     if rc:
         raise RuntimeError(f"oracle recursive count failed: {rc}")
     return w, nn
+
+
+PLANT_BASE, PLANT_JMARK = 2, 4
+
+
+def count_records_13py(N: int, rec, plant: int = 0):
+    """(weighted, nodes) under the pruning of the reference's recursive SQ*
+    functions (13Bit_codon/13Py_constellations_codon.py:183-678).  plant
+    (tests only): PLANT_BASE counts every base frame as a node, PLANT_JMARK
+    counts column 0 before the jmark mask."""
+    rec = np.ascontiguousarray(np.asarray(rec, dtype=np.uint32))
+    m = rec.shape[0]
+    w = np.zeros(m, np.uint64)
+    nn = np.zeros(m, np.uint64)
+    rc = lib().nqo_count_records_13py(N, rec, m, w, nn, int(plant))
+    if rc:
+        raise RuntimeError(f"oracle 13Py-pruning count failed: {rc}")
+    return w, nn
+
+
+class gate_bits:
+    """with oracle.gate_bits(w): build_soa's rd-word width for the j-diagonal
+    gate (32 in the reference); records past it take the P5 functions."""
+
+    def __init__(self, w: int):
+        self.w = int(w)
+
+    def __enter__(self):
+        lib().nqo_set_gate_bits(self.w)
+        return self
+
+    def __exit__(self, *exc):
+        lib().nqo_set_gate_bits(32)
 
 
 def build_soa(N: int, rec) -> np.ndarray:

@@ -6,7 +6,14 @@ Pins (see oracle/nq_oracle.c):
   * totals: A000170 (18Py:2089); unique: README.en:12-33 through Burnside
   * F90/F180 and unique: brute-force enumeration for N <= 12
   * preset-6/7 record counts from the reference's logs
-  * node counts: iterative and recursive restatements agree per record
+  * node counts: iterative and recursive restatements agree per record;
+    under the pruning of the reference's recursive SQ* functions the oracle
+    reproduces that solver's own traced node counts record by record
+    (nodes_13py.npz, N=6..14), and planted accounting mistakes break that
+  * P5 functionids: with build_soa's j-diagonal gate narrowed to W >= N
+    bits, every record past it takes the P5 branch and the per-record counts
+    are unchanged; the 18 P5 records of N=24 preset 5 count what their
+    non-P5 descendants two queens deeper count (p5_n24.npz)
   * the reference's own C solver (oracle/_ref) at preset 4
 """
 import numpy as np
@@ -133,3 +140,119 @@ def test_pins_detect_planted_mistakes(oracle, p5_full, digests):
     dropped = w.copy()
     dropped[np.argmax(w)] -= 1
     assert int(dropped.sum()) != digests[str(N)]["solutions_sum"]
+
+
+@pytest.fixture(scope="module")
+def census():
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "nodes_13py.npz") as z:
+        return {k: z[k] for k in z.files}
+
+
+def _census_cases(census):
+    return sorted(k[3:] for k in census if k.startswith("rec"))
+
+
+def test_node_counts_match_reference_census(oracle, census):
+    """The reference's recursive solver (13Py, traced line by line) against
+    the oracle's node accounting with the same pruning: equal node counts for
+    every record of every case, and equal weighted counts except where 13Py
+    counts a root on the terminal row before testing its free set (N=6,
+    preset 5: the 18Py fixture has those as 0)."""
+    assert len(_census_cases(census)) == 17
+    for tag in _census_cases(census):
+        N = int(tag.split("_")[0])
+        rec = census["rec" + tag]
+        w, nodes = oracle.count_records_13py(N, rec)
+        assert np.array_equal(nodes, census["nodes" + tag]), tag
+        bad = np.nonzero(w != census["sol" + tag])[0]
+        if tag == "6_5":
+            soa = oracle.build_soa(N, rec[bad])
+            assert len(bad) and (soa[:, 7] == 0).all() and (soa[:, 1] == soa[:, 3]).all()
+        else:
+            assert len(bad) == 0, tag
+            w18, _, _ = oracle.count_records(N, rec)
+            assert np.array_equal(w18, w), tag
+
+
+@pytest.mark.parametrize("plant", ["base", "jmark"])
+def test_planted_node_mistakes_fail_the_census(oracle, census, plant):
+    """Counting a base frame as a node, or column 0 before the jmark mask,
+    changes node counts the reference's census fixes."""
+    flag = {"base": oracle.PLANT_BASE, "jmark": oracle.PLANT_JMARK}[plant]
+    wrong = 0
+    for tag in _census_cases(census):
+        N = int(tag.split("_")[0])
+        rec = census["rec" + tag]
+        w, nodes = oracle.count_records_13py(N, rec, plant=flag)
+        wrong += int((nodes != census["nodes" + tag]).sum())
+        if tag != "6_5":
+            assert np.array_equal(w, census["sol" + tag])   # the count is untouched
+    assert wrong > 100
+
+
+def test_node_definitions_differ_only_on_pruned_functionids(oracle, census):
+    """The product's node definition (18Py's availptn table) and 13Py's
+    differ only through SQBjrB (fid 2, reached from targets 0..7) and the
+    SQd1BlB mark step (fid 20, from targets 19 and 20): records of the other
+    targets have equal node counts under both."""
+    for tag in _census_cases(census):
+        N = int(tag.split("_")[0])
+        rec = census["rec" + tag]
+        fid = oracle.build_soa(N, rec)[:, 0]
+        same = ~np.isin(fid, list(range(0, 8)) + [19, 20])
+        _, _, nodes = oracle.count_records(N, rec)
+        assert np.array_equal(nodes[same], census["nodes" + tag][same]), tag
+
+
+@pytest.mark.parametrize("N", range(9, 17))
+def test_p5_branch_under_narrow_gate(oracle, digests, N):
+    """build_soa's gate sets the j queen's rd bit at the start row only when
+    it fits the rd word (j >= 2N-33-start at 32 bits); other records take the
+    P5 functions, which set it on the row it enters the board.  Narrowing the
+    word to W bits (W >= N) moves records onto the P5 branch without changing
+    any count: the per-record weighted counts keep the reference fixture's
+    digest, the iterative and recursive restatements agree on counts and
+    nodes, and all four P5 functionids occur.  At W = N-1 the gate also cuts
+    the j >= N-3 families, which have no P5 branch, and the total is wrong."""
+    from nqueens import constellations as C
+    from tests.helpers import solutions_sha256
+    rec = C.gen_constellations(N, 5)
+    d = digests[str(N)]
+    fids = set()
+    for W in (N, N + 1, N + 3):
+        with oracle.gate_bits(W):
+            fid = oracle.build_soa(N, rec)[:, 0]
+            w, _, nodes = oracle.count_records(N, rec)
+            w2, nodes2 = oracle.count_records_recursive(N, rec)
+        fids |= set(fid[(fid >= 8) & (fid <= 11)].tolist())
+        assert int(w.sum()) == d["solutions_sum"], W
+        assert solutions_sha256(rec, w) == d["solutions_sha256"], W
+        assert np.array_equal(w, w2) and np.array_equal(nodes, nodes2), W
+    assert fids == {8, 9, 10, 11} or N < 12
+    with oracle.gate_bits(N - 1):
+        assert int(oracle.count_records(N, rec)[0].sum()) != d["solutions_sum"]
+
+
+def test_p5_records_of_n24_partition(oracle):
+    """The 18 P5 records of N=24 preset 5 (cached oracle counts): each counts
+    what its descendants two preset queens deeper (no P5 among them) count,
+    and its node count is theirs plus one per extraction on the two expanded
+    rows (placed without the lookahead); the records, their P5 targets and
+    their expansions are recomputed here."""
+    from nqueens import constellations as C
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "p5_n24.npz") as z:
+        f = {k: z[k] for k in z.files}
+    rec = C.gen_constellations(24, 5)
+    fid = oracle.build_soa(24, rec)[:, 0]
+    idx = np.nonzero((fid >= 8) & (fid <= 11))[0]
+    assert np.array_equal(idx, f["index"]) and np.array_equal(rec[idx], f["rec"]) and len(idx) == 18
+    assert np.array_equal(f["w"], f["kids_w"])
+    assert np.array_equal(f["nodes"], f["kids_nodes"] + f["kids6"] + f["kids"])
+    for a in range(len(idx)):
+        assert oracle.expand_records(24, f["rec"][a:a + 1], 1).shape[0] == f["kids6"][a]
+        k7 = oracle.expand_records(24, f["rec"][a:a + 1], 2)
+        assert k7.shape[0] == f["kids"][a]
+        kf = oracle.build_soa(24, k7)[:, 0]
+        assert not ((kf >= 8) & (kf <= 11)).any()
