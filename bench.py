@@ -60,6 +60,7 @@ def main() -> int:
     ap.add_argument("--scheduler", choices=["fifo", "mlp"], default="mlp")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--big-n", type=int, default=21, help="also time one exact count of this N (0 = skip)")
+    ap.add_argument("--big-plain", type=int, default=1, help="time the plain kernel on the --big-n records too")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration (0 = skip)")
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
@@ -71,9 +72,9 @@ def main() -> int:
 
     import nqueens
     from nqueens import autotune
-    known = json.loads((ROOT / "tests" / "golden" / "known.json").read_text())
+    from nqueens.expected import TOTAL, UNIQUE
     N, preset = args.n, args.preset
-    expect_total, expect_unique = known["readme_total_unique"][str(N)]
+    expect_total, expect_unique = TOTAL[N], UNIQUE[N]
 
     ctx = nqueens.Context(0)
     info = ctx.device_info()
@@ -137,6 +138,30 @@ def main() -> int:
     log(f"[bench] batched {ms_per_step:.2f} ms/count, {value / 1e9:.2f} G nodes/s, "
         f"device run {np.mean(run_ms):.2f} ms")
 
+    # the same schedule in generation (FIFO) order: what the cost model's
+    # device ordering adds to the batched kernel
+    fifo = None
+    if params is not None:
+        ctx.load(N, rec)
+        tot, _ = ctx.run_batched(sched)
+        assert tot == expect_total
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tot, _ = ctx.run_batched(sched)
+            assert tot == expect_total
+        fwall = (time.perf_counter() - t0) / args.steps
+        fifo = {"nodes_per_s": round(nodes_total / fwall, 1), "ms_per_count": round(1000 * fwall, 3),
+                "cost_model_speedup": round(fwall / (wall / args.steps), 4)}
+        log(f"[bench] fifo order {1000 * fwall:.2f} ms/count")
+
+    # one whole count from scratch as a caller makes it: host generation,
+    # load and decode, cost order, batched run, Unique (nqueens.count)
+    t0 = time.perf_counter()
+    ct, cu = nqueens.count(N, preset=preset, scheduler=args.scheduler, ctx=ctx)
+    count_wall = time.perf_counter() - t0
+    assert (ct, cu) == (expect_total, expect_unique)
+    log(f"[bench] nqueens.count({N}) from scratch {count_wall * 1000:.1f} ms")
+
     # plain kernel on the same records (record order)
     ctx.load(N, rec)
     tot, _ = ctx.run_plain()
@@ -174,7 +199,7 @@ def main() -> int:
 
     big = None
     if args.big_n:
-        big = big_count(nqueens, ctx, known, args.big_n, preset, args.scheduler)
+        big = big_count(nqueens, ctx, args.big_n, preset, args.scheduler, args.big_plain)
         log(f"[bench] N={args.big_n}: {big}")
 
     cpu = None
@@ -185,7 +210,7 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
-        "scaling": "strong", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "scaling": "strong", "vs_baseline": None, "dtype": "u32", "data": "real",
         "config": {"workload": f"exact count N={N}, preset {preset}: {rec.shape[0]} constellations, "
                                f"batched persistent kernel, {args.scheduler} order",
                    "N": N, "preset": preset, "constellations": int(rec.shape[0]), "work_items": items,
@@ -194,6 +219,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "total": expect_total, "unique": unique, "nodes": nodes_total,
         "wall_s_per_count": round(ms_per_step / 1000, 5), "run_ms_device": round(float(np.mean(run_ms)), 3),
+        "count_wall_s": round(count_wall, 4), "fifo": fifo,
         "plain": {"nodes_per_s": round(plain_value, 1), "ms_per_count": round(1000 * plain_wall, 3),
                   "kernel_ms": round(float(np.mean(pk)), 3)},
         "speedup_vs_plain": round(value / plain_value, 3),
@@ -220,10 +246,12 @@ def profiled_traffic(N: int, preset: int, sched):
     return None
 
 
-def big_count(nqueens, ctx, known, N: int, preset: int, scheduler: str) -> dict:
+def big_count(nqueens, ctx, N: int, preset: int, scheduler: str, plain: bool) -> dict:
     """One exact count of a larger N (wall time of the count call and its
-    node rate; nodes from a node-counting run of the same set)."""
+    node rate; nodes from a node-counting run of the same set), and the plain
+    one-lane-per-constellation kernel on the same records."""
     from nqueens import autotune
+    from nqueens.expected import TOTAL
     rec = nqueens.gen_constellations(N, preset)
     ctx.load(N, rec)
     params = None
@@ -240,13 +268,24 @@ def big_count(nqueens, ctx, known, N: int, preset: int, scheduler: str) -> dict:
         ctx.order_by_cost(params)
     tot2, t = ctx.run_batched(s)
     wall = time.perf_counter() - t0
-    want = known["readme_total_unique"][str(N)][0]
+    want = TOTAL[N]
     if tot != want or tot2 != want:
         raise SystemExit(f"N={N}: wrong total {tot}/{tot2} != {want}")
     f90, f180 = ctx.symmetric_counts(N)
-    return {"N": N, "preset": preset, "total": tot2, "unique": nqueens.unique_count(tot2, f90, f180),
-            "constellations": int(rec.shape[0]), "nodes": int(nodes.sum()), "wall_s": round(wall, 4),
-            "nodes_per_s": round(float(nodes.sum()) / wall, 1), "split_depth": s.split_depth, "block": s.block}
+    out = {"N": N, "preset": preset, "total": tot2, "unique": nqueens.unique_count(tot2, f90, f180),
+           "constellations": int(rec.shape[0]), "nodes": int(nodes.sum()), "wall_s": round(wall, 4),
+           "nodes_per_s": round(float(nodes.sum()) / wall, 1), "split_depth": s.split_depth, "block": s.block}
+    if plain:
+        ctx.load(N, rec)
+        t0 = time.perf_counter()
+        tp, tt = ctx.run_plain()
+        pwall = time.perf_counter() - t0
+        if tp != want:
+            raise SystemExit(f"N={N} plain kernel: wrong total {tp} != {want}")
+        out["plain"] = {"wall_s": round(pwall, 3), "kernel_ms": round(tt.kernel_ms, 1),
+                        "nodes_per_s": round(float(nodes.sum()) / pwall, 1)}
+        out["speedup_vs_plain"] = round(pwall / wall, 3)
+    return out
 
 
 def cpu_baseline(N: int, rec: np.ndarray, nodes: np.ndarray, seconds: float) -> dict:

@@ -32,9 +32,9 @@ extern "C" {
 
 enum nq_status {
     NQ_OK = 0,
-    NQ_EINVAL = -1,        /* bad argument (N out of 6..24, NULL pointer, size) */
-    NQ_EUNSUPPORTED = -2,  /* record outside the u32 path: P5 target (fids 8-11),
-                              shift out of range, no terminal row */
+    NQ_EINVAL = -1,        /* bad argument (N out of 6..28, NULL pointer, size) */
+    NQ_EUNSUPPORTED = -2,  /* record outside the u32 path: shift out of range,
+                              no terminal row, terminal depth beyond 16 */
     NQ_ECAPACITY = -3,     /* DFS stack / work-item buffer capacity exceeded */
     NQ_EHIP = -4,          /* HIP runtime error (message in nq_last_error) */
     NQ_ENOMEM = -5,
@@ -151,7 +151,7 @@ int nq_count_noalloc(nq_ctx *ctx, int N, const uint32_t *rec4, uint64_t m, const
  * orbit count of 13Bit_codon/07Py_bit_symmetry_codon.py:410-411
  * (unique = c2+c4+c8).  This returns the number of solutions fixed by the
  * quarter turn (F90) and by the half turn (F180); the host forms
- * unique = (total + F180 + 2*F90) / 8. */
+ * unique = (total + F180 + 2*F90) / 8.  2 <= N <= 27. */
 int nq_symmetric_counts(nq_ctx *ctx, int N, uint64_t *f90_out, uint64_t *f180_out);
 
 /* ---- small boards ----------------------------------------------------------

@@ -16,8 +16,7 @@ namespace nqk {
 // per-record schedule (layout: nqt::sched_words)
 // ---------------------------------------------------------------------------
 struct Sched {
-    uint32_t t[4];            // byte r (1..15): parameter byte of depth T-r
-    uint32_t q0;              // parameter byte of depth 0
+    uint32_t t[4];            // byte r (0..15): parameter byte of depth T-1-r
     uint32_t T;               // terminal depth (<= 16)
     uint32_t tm;              // terminal free mask
     uint32_t mask;            // board mask (1 << N) - 1
@@ -34,7 +33,7 @@ NQ_HD inline uint32_t table_byte(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t
 // Parameter byte of depth d < T.
 NQ_HD inline uint32_t qbyte(const Sched &S, uint32_t d)
 {
-    return d == 0u ? S.q0 : table_byte(S.t[0], S.t[1], S.t[2], S.t[3], S.T - d);
+    return table_byte(S.t[0], S.t[1], S.t[2], S.t[3], S.T - 1u - d);
 }
 
 NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
@@ -43,7 +42,6 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
     Sched S;
     S.t[0] = a.x; S.t[1] = a.y; S.t[2] = a.z; S.t[3] = a.w;
     S.T = i.x & 0xFFu;
-    S.q0 = i.x >> 24;
     S.tm = i.y;
     const uint32_t N = (i.x >> 16) & 31u;
     S.mask = (1u << N) - 1u;
@@ -54,27 +52,24 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // ---------------------------------------------------------------------------
 // frames
 // ---------------------------------------------------------------------------
-// A frame is {ld, rd, col, W}.  The frame word W holds the candidates
-// (avail, bits 0-27) and r = T-1-depth (bits 28-31): the frame's children
-// are on the terminal row exactly when r = 0, i.e. when W is below 2^28, and
-// a child's word is W's top bits less 2^28 plus its candidates.  col holds
-// the occupied columns in bits 0-23 (N <= 24) and the parameter byte of the
-// frame's depth in bits 24-31; the board mask drops that byte wherever col
-// meets a board set.
+// A frame is {ld, rd, col, W}: the diagonals, the occupied columns (board
+// bits 0..N-1, N <= 28) and the frame word W, which holds the candidates
+// (avail, bits 0-27) and r = T-1-depth (bits 28-31).  The frame's children
+// are on the terminal row exactly when r = 0, i.e. when W is below 2^28; a
+// child's word is W's top bits less 2^28 plus its candidates; and the
+// frame's parameter byte is byte r of the lane's schedule table, looked up
+// from W by each step (frame_q).
 constexpr uint32_t kAvailMask = 0x0FFFFFFFu;
 constexpr uint32_t kQShift = 24u;
 constexpr uint32_t kRShift = 28u;
-constexpr uint32_t kColMask = 0x00FFFFFFu;
 
 NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t r) { return avail | (r << kRShift); }
-NQ_HD inline uint32_t frame_col(uint32_t col, uint32_t q) { return col | (q << kQShift); }
 
-// Frame word and col of a frame at depth d with candidates avail.
+// Frame word of a frame at depth d with candidates avail.
 NQ_HD inline uint32_t start_word(const Sched &S, uint32_t avail, uint32_t d)
 {
     return frame_word(avail, S.T - 1u - d);
 }
-NQ_HD inline uint32_t start_col(const Sched &S, uint32_t col, uint32_t d) { return frame_col(col, qbyte(S, d)); }
 
 // Any function of three words, bit by bit: bit i of IMM is the result for
 // the input bits (a, b, c) = (i >> 2, i >> 1, i) & 1.  On gfx950 this is one
@@ -110,30 +105,32 @@ NQ_HD inline uint32_t vconst(uint32_t v)
 }
 
 // Per-lane constants of the walk: board mask, rd shift, the parameter bytes
-// by r and the column field mask of col (in a VGPR).  An idle lane (mask 0)
-// steps without effect: no candidate, no count, no stack motion.
-// The rd block constant ARc << (N-4) comes straight from col: ARc sits in
-// col's bits 28-29, so (col >> (32-N)) & (3 << (N-4)) is it.
+// by r and the masks below (in VGPRs).  An idle lane (mask 0) steps without
+// effect: no candidate, no count, no stack motion.
+// With the frame's parameter byte q in bits 24-31 of a word, the rd constant
+// of the child is (q >> (32-N)) & km, km = 0b1011 << (N-4): ARc (bits 28-29)
+// to N-4.., JD (bit 31) to N-1.
+constexpr uint32_t kNlaBit = 1u << 30;   // the NLA flag of q
 struct LaneK {
     uint32_t mask;
-    uint32_t ksh, km;     // 32 - N and 3 << (N-4)
+    uint32_t ksh, km;     // 32 - N and 0b1011 << (N-4)
     uint32_t t0, t1, t2, t3;
-    uint32_t cm;
+    uint32_t nlab, three;
 };
 NQ_HD inline LaneK lane_k(uint32_t N, uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3)
 {
-    return LaneK{(1u << N) - 1u, 32u - N, 3u << (N - 4u), t0, t1, t2, t3, vconst(kColMask)};
+    return LaneK{(1u << N) - 1u, 32u - N, 0xBu << (N - 4u), t0, t1, t2, t3, vconst(kNlaBit), vconst(3u)};
 }
 NQ_HD inline LaneK lane_consts(const Sched &S) { return lane_k(S.nsh + 4u, S.t[0], S.t[1], S.t[2], S.t[3]); }
-NQ_HD inline LaneK idle_lane() { return LaneK{0u, 28u, 3u, 0u, 0u, 0u, 0u, vconst(kColMask)}; }
+NQ_HD inline LaneK idle_lane() { return LaneK{0u, 28u, 3u, 0u, 0u, 0u, 0u, vconst(kNlaBit), vconst(3u)}; }
 
-// Parameter byte of the children of a frame, in bits 24-31 (bits 0-23 are
-// not defined), from the frame word (top bits: the frame's r).  On the
-// device two v_perm_b32 place the byte from each register pair in byte 3:
-// the selector is W's bits 4-30, whose byte 3 is r & 7 (its other bytes
-// select arbitrary bytes); the sign of W (r >= 8) picks the pair.  An
-// indexed select over the words would be lowered to a scratch array.
-NQ_HD inline uint32_t child_q(const LaneK &L, uint32_t W)
+// Parameter byte of a frame, in bits 24-31 (bits 0-23 are not defined), from
+// its frame word (top bits: the frame's r).  On the device two v_perm_b32
+// place the byte from each register pair in byte 3: the selector is W's bits
+// 4-30, whose byte 3 is r & 7 (its other bytes select arbitrary bytes); the
+// sign of W (r >= 8) picks the pair.  An indexed select over the words would
+// be lowered to a scratch array.
+NQ_HD inline uint32_t frame_q(const LaneK &L, uint32_t W)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const uint32_t sel = __builtin_amdgcn_ubfe(W, 4u, 27u);
@@ -205,9 +202,10 @@ struct Walk {
 
 // One DFS node, without divergent branches.  Every lane extracts one
 // candidate of its frame and builds the child (18Py:368-562 one iteration):
-// shift by the step, OR the block constants, drop column 0 when the child is
-// the jmark frame, apply the one-row lookahead.  A child on the terminal row
-// (depth T) counts one solution when its free set is not empty.
+// shift by the step, OR the block constants (the jmark's ld bit and the P5
+// j diagonal among them), apply the one-row lookahead to the shifted ld
+// before the constants.  A child on the terminal row (depth T) counts one
+// solution when its free set is not empty.
 // The lane then descends (pushing the frame's remainder when it has one),
 // keeps the remainder, or pops.  The top entry is read every step (used only
 // when popping), so its load is issued a full step before its use.  Frames
@@ -215,46 +213,38 @@ struct Walk {
 // above the inert frame's.  Returns true when the walk's subtree is
 // exhausted (the step took the inert frame).
 // ISSUE: spend two full-rate VALU ops where one slower op would do (the
-// no-prune sign, the child's frame word).  That pays where the SIMD is
-// issue-bound (the persistent kernel, 16 waves per CU) and costs where a
-// lane's dependency chain is the limit (one lane per constellation).
+// child's frame word).  That pays where the SIMD is issue-bound (the
+// persistent kernel, 16 waves per CU) and costs where a lane's dependency
+// chain is the limit (one lane per constellation).
 template <bool NODES, bool ISSUE = true, class Count, class Stack>
 NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint64_t &nodes)
 {
     const uint32_t S = st.S;
     const uint4 top = st.read(w.A);
     const uint32_t W = w.W, C = w.col;
+    const uint32_t q = frame_q(L, W);                    // this frame's parameter byte (bits 24-31)
     const uint32_t bit = W & (0u - W);                   // lowest candidate (avail is the low field)
     const uint32_t rest = W ^ bit;
-    const uint32_t s = (C >> 24) & 3u;
-    const uint32_t nld = ((w.ld | bit) << s) | ((C >> 26) & 3u);
-    const uint32_t nrd = bop3<kOrAnd>((w.rd | bit) >> s, C >> L.ksh, L.km);
+    const uint32_t s = (q >> 24) & 3u;
+    const uint32_t x = (w.ld | bit) << s;                // the child's ld before its constants
+    const uint32_t nld = bop3<kOrAnd>(x, q >> 26, L.three);          // | AL
+    const uint32_t nrd = bop3<kOrAnd>((w.rd | bit) >> s, q >> L.ksh, L.km);
     const uint32_t ncol = C | bit;
-    const uint32_t occ = bop3<kOr3>(nld, nrd, C >> 31);  // JMN: column 0 of a jmark child
-    uint32_t nf = bop3<kAndNotOr>(L.mask, occ, ncol);
+    uint32_t nf = bop3<kAndNotOr>(L.mask, nld | nrd, ncol);
+    // nonzero when the frame has no prune
+    const uint32_t nla = q & L.nlab;
 #if defined(__HIP_DEVICE_COMPILE__)
     asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
-    // all ones when the frame has no prune
-    uint32_t nla;
-    if constexpr (ISSUE) {   // C << 1 as an add, then the sign (not one v_bfe_i32)
-        uint32_t c2;
-        asm("v_add_u32 %0, %1, %1" : "=v"(c2) : "v"(C));
-        nla = (uint32_t)((int32_t)c2 >> 31);
-    } else {
-        nla = (uint32_t)((int32_t)(C << 1) >> 31);
-    }
-    uint32_t nld2;
-    // nld << 1 as an add: v_add_u32 issues at the full rate, v_lshlrev_b32
+    uint32_t x2;
+    // x << 1 as an add: v_add_u32 issues at the full rate, v_lshlrev_b32
     // at half (the compiler turns x + x back into a shift)
-    asm("v_add_u32 %0, %1, %1" : "=v"(nld2) : "v"(nld));
+    asm("v_add_u32 %0, %1, %1" : "=v"(x2) : "v"(x));
 #else
-    const uint32_t nla = (uint32_t)((int32_t)(C << 1) >> 31);
-    const uint32_t nld2 = nld << 1;
+    const uint32_t x2 = x << 1;
 #endif
-    const uint32_t look = bop3<kOr3>(nld2, nrd >> 1, ncol);
+    const uint32_t look = bop3<kOr3>(x2, nrd >> 1, ncol);
     const bool lok = bop3<kAndNotOrC>(L.mask, look, nla) != 0u;
     const uint32_t hiW = W & ~kAvailMask;                // r << 28
-    const uint32_t qc = child_q(L, W);
     const bool inner = W >= (1u << kRShift);             // children not on the terminal row
     const bool ok = lok & (nf != 0u);
     const bool down = ok & inner;
@@ -277,10 +267,9 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     if constexpr (ISSUE)   // two two-operand adds, not a v_add3 with a scalar operand
         asm("v_add_u32 %0, %1, %2" : "=v"(Wc) : "v"(hiW - (1u << kRShift)), "v"(nf));
 #endif
-    const uint32_t colc = bop3<kSel>(L.cm, ncol, qc);    // columns from ncol, parameter byte from qc
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
-    w.col = down ? colc : (up ? top.z : C);
+    w.col = down ? ncol : (up ? top.z : C);
     w.W = down ? Wc : (up ? top.w : rest);
     w.A += push ? S : (pop ? 0u - S : 0u);
     // written after the selects: the wait for the top entry does not also
@@ -317,12 +306,13 @@ NQ_HD inline void expand_subtree(const Sched &S, uint32_t ld, uint32_t rd, uint3
         const uint32_t bit = avail & (0u - avail);
         avail ^= bit;
         const uint32_t s = q & 3u;
-        const uint32_t nld = ((ld | bit) << s) | ((q >> 2) & 3u);
-        const uint32_t nrd = ((rd | bit) >> s) | (((q >> 4) & 3u) << S.nsh);
+        const uint32_t x = (ld | bit) << s;
+        const uint32_t nld = x | ((q >> 2) & 3u);
+        const uint32_t nrd = ((rd | bit) >> s) | (((q >> 4) & 3u) << S.nsh) | (((q >> 7) & 1u) << (S.nsh + 3u));
         const uint32_t ncol = col | bit;
-        const uint32_t occ = nld | nrd | ncol | ((q >> 7) & 1u);
+        const uint32_t occ = nld | nrd | ncol;
         const uint32_t nf = S.mask & ~occ;
-        const bool lok = ((q >> 6) & 1u) != 0u || (S.mask & ~((nld << 1) | (nrd >> 1) | ncol)) != 0u;
+        const bool lok = ((q >> 6) & 1u) != 0u || (S.mask & ~((x << 1) | (nrd >> 1) | ncol)) != 0u;
         const uint32_t cd = d + 1u;
         if (NODES) nodes += 1u;
         bool down = false;
@@ -333,15 +323,15 @@ NQ_HD inline void expand_subtree(const Sched &S, uint32_t ld, uint32_t rd, uint3
             else down = true;
         }
         if (down) {
-            if (avail) { st.put(sp, make_uint4(ld, rd, col, avail | (d << 24))); sp++; }
+            if (avail) { st.put(sp, make_uint4(ld, rd, col, avail | (d << kRShift))); sp++; }
             ld = nld; rd = nrd; col = ncol; avail = nf; d = cd; q = qbyte(S, cd);
         } else if (avail == 0u) {
             if (sp == 0) break;
             sp--;
             const uint4 e = st.get(sp);
             ld = e.x; rd = e.y; col = e.z;
-            avail = e.w & kColMask;   // avail < 2^22 sits below the depth byte
-            d = e.w >> 24;
+            avail = e.w & kAvailMask;   // avail < 2^28 sits below the depth
+            d = e.w >> kRShift;
             q = qbyte(S, d);
         }
     }

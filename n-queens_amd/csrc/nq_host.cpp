@@ -375,7 +375,7 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
     uint32_t err = 0;
     NQ_HIPCHK(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
     NQ_HIPCHK(hipStreamSynchronize(c->stream));
-    if (err) return nq_fail(NQ_EUNSUPPORTED, "record decode failed (status mask 0x%x: 2=P5 target, 4=shift, "
+    if (err) return nq_fail(NQ_EUNSUPPORTED, "record decode failed (status mask 0x%x: 2=P5 record past its switch row, 4=shift, "
                                           "8=no terminal row, 16=depth)", err);
     // the terminal depth is N-1-preset for every record of a set; take the max
     c->Tmax = 0;
@@ -584,7 +584,7 @@ int nq_count(nq_ctx *c, int N, const uint32_t *rec4, uint64_t m, const nq_sched 
 int nq_symmetric_counts(nq_ctx *c, int N, uint64_t *f90_out, uint64_t *f180_out)
 {
     if (int r = check_ctx(c)) return r;
-    if (N < 2 || N > 25) return nq_fail(NQ_EINVAL, "N=%d outside 2..25", N);
+    if (N < 2 || N > 27) return nq_fail(NQ_EINVAL, "N=%d outside 2..27", N);
     NQ_HIPCHK(nqk::launch_fill_u64(c->stream, c->sym.p, 2, 0));
     NQ_HIPCHK(nqk::launch_symmetric(c->stream, N, c->sym.p));
     unsigned long long v[2] = {0, 0};

@@ -33,7 +33,7 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
-    Walk w{root.x, root.y, start_col(S, root.z, 0u), start_word(S, root.w, 0u), st.empty()};
+    Walk w{root.x, root.y, root.z, start_word(S, root.w, 0u), st.empty()};
     while (!walk_step<NODES, false>(L, w, st, cnt, nodes)) {   // one lane per record: latency-bound
     }
 }
@@ -130,11 +130,11 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     if (r.w != 0u) {
         if (S.T < 2u || split == 0u || split > S.T - 2u) {
             // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
-            if (S.T) put(r.x, r.y, start_col(S, r.z, 0u), start_word(S, r.w, 0u));
+            if (S.T) put(r.x, r.y, r.z, start_word(S, r.w, 0u));
             else put(r.x, r.y, r.z, r.w);
         } else {
-            const uint32_t upw = start_word(S, 0u, split), upc = start_col(S, 0u, split);
-            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc | upc, f | upw); };
+            const uint32_t upw = start_word(S, 0u, split);
+            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | upw); };
             expand_subtree<NODES>(S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
         }
     }
@@ -418,7 +418,11 @@ __device__ __forceinline__ uint32_t row_free(int N, int r, uint32_t full, uint32
     return f;
 }
 
-__device__ __forceinline__ bool quarter_turn(int N, int h, uint64_t q)
+// Columns of the rows r < h = N/2 of a half-turn symmetric placement, 5 bits
+// each (13 rows for N <= 27).
+typedef unsigned __int128 qrows_t;
+
+__device__ __forceinline__ bool quarter_turn(int N, int h, qrows_t q)
 {
     // q holds the column of row r < h in bits 5r..5r+4; rows >= N-h mirror them
     for (int r = 0; r < N; r++) {
@@ -443,7 +447,8 @@ __global__ void k_symmetric(int N, int P, uint32_t nprefix, unsigned long long *
     if (g < nprefix) {
         const uint32_t full = (1u << N) - 1u;
         uint32_t cols = 0;
-        uint64_t dg = 0, an = 0, q = 0;
+        uint64_t dg = 0, an = 0;
+        qrows_t q = 0;
         if (N & 1) {
             const int mid = N / 2;
             cols = 1u << mid;
@@ -460,7 +465,7 @@ __global__ void k_symmetric(int N, int P, uint32_t nprefix, unsigned long long *
             cols |= (1u << c) | (1u << (N - 1 - c));
             dg |= (1ull << (c - r + N - 1)) | (1ull << (r - c + N - 1));
             an |= (1ull << (r + c)) | (1ull << (2 * N - 2 - r - c));
-            q |= (uint64_t)c << (5 * r);
+            q |= (qrows_t)c << (5 * r);
         }
         if (ok) {
             if (P == h) {
@@ -470,7 +475,8 @@ __global__ void k_symmetric(int N, int P, uint32_t nprefix, unsigned long long *
                 // iterative DFS over rows P..h-1
                 uint32_t fst[16];
                 uint32_t cst[16];
-                uint64_t dst[16], ast[16], qst[16];
+                uint64_t dst[16], ast[16];
+                qrows_t qst[16];
                 int r = P;
                 uint32_t f = row_free(N, r, full, cols, dg, an);
                 for (;;) {
@@ -486,7 +492,7 @@ __global__ void k_symmetric(int N, int P, uint32_t nprefix, unsigned long long *
                     const uint32_t ncols = cols | bit | (1u << (N - 1 - c));
                     const uint64_t ndg = dg | (1ull << (c - r + N - 1)) | (1ull << (r - c + N - 1));
                     const uint64_t nan_ = an | (1ull << (r + c)) | (1ull << (2 * N - 2 - r - c));
-                    const uint64_t nq = q | ((uint64_t)c << (5 * r));
+                    const qrows_t nq = q | ((qrows_t)c << (5 * r));
                     if (r + 1 == h) {
                         c180++;
                         if (quarter_turn(N, h, nq)) c90++;

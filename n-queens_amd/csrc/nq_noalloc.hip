@@ -128,7 +128,7 @@ extern "C" int nq_count_noalloc(nq_ctx *c, int N, const uint32_t *rec4, uint64_t
     uint32_t hs[4] = {0, 0, 0, 0};
     NQ_HIPCHK(hipMemcpyAsync(hs, small, sizeof hs, hipMemcpyDeviceToHost, st));
     NQ_HIPCHK(hipStreamSynchronize(st));
-    if (hs[0]) return nq_fail(NQ_EUNSUPPORTED, "record decode failed (status mask 0x%x: 2=P5 target, 4=shift, "
+    if (hs[0]) return nq_fail(NQ_EUNSUPPORTED, "record decode failed (status mask 0x%x: 2=P5 record past its switch row, 4=shift, "
                                                "8=no terminal row, 16=depth)", hs[0]);
     r.Tmin = hs[2];
     r.Tmax = hs[3];

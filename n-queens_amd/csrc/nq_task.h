@@ -12,16 +12,21 @@
 //
 //   bits 0-1  s     rows advanced by a placement (1, or 2/3 on a mark row)
 //   bits 2-3  AL    constant OR-ed into the child's ld after the shift
-//                   (blockL | add1); on the jmark frame AL = 2 instead, which
-//                   is the reference's "ld |= 1" on that frame shifted by its
-//                   step of 1 (a jmark frame is never a mark frame)
+//                   (blockL | add1); bit 0 is also set when the child is the
+//                   jmark frame: the reference's "ld |= 1" on that frame
+//                   (18Py:419-421), which takes column bit 0 from its
+//                   candidates (its own children then see it shifted by
+//                   their step; a jmark frame is never a mark frame); and on
+//                   depth T-1 when the terminal functionid is 14 (see below).
+//                   The one-row lookahead is computed before AL is OR-ed in,
+//                   as the reference's parent does not see the jmark's bit
 //   bits 4-5  ARc   constant OR-ed into the child's rd: ARc << (N-4), i.e.
 //                   0, 1<<(N-4) or 1<<(N-3)
 //   bit  6    NLA   no one-row lookahead prune at this depth (the prune runs
 //                   when availptn is set and row+1 < endmark)
-//   bit  7    JMN   the child is the jmark frame: column bit 0 is not one of
-//                   its candidates (18Py:419-421); also set on depth T-1 when
-//                   the terminal functionid is 14 (see below)
+//   bit  7    JD    the child is the P5 switch frame: its rd gets bit N-1,
+//                   the j queen's down-right diagonal entering the board
+//                   (see the P5 note in decode)
 //
 // T is the depth of the terminal row (row == endmark on a base functionid);
 // a child created at depth T counts one solution when its free set is
@@ -37,14 +42,14 @@ namespace nqt {
 
 constexpr int kMaxDepth = 24;        // parameter bytes per descriptor (6 words)
 constexpr int kMinN = 6;
-// u32 masks: the board and col fit 24 bits (col's top byte carries the
-// walk's parameter byte); build_soa's rd bit (N-1-j) + (N-2-start) is set
-// only when j >= 2N-33-start, which keeps it at bit 30 or below
-constexpr int kMaxN = 24;
+// u32 masks: the board fits the walk's 28-bit candidate field; build_soa's
+// rd bit (N-1-j) + (N-2-start) is set only when j >= 2N-33-start, which keeps
+// it at bit 30 or below (the P5 records set it later)
+constexpr int kMaxN = 28;
 
 enum : int32_t {
     kOk = 0,
-    kErrP5 = 1,          // target functionid 8..11 (none for N <= 23 at presets 5-6, N = 24 at preset 7)
+    kErrP5 = 1,          // a P5 record whose switch row is not reached (never for a canonical record)
     kErrShift = 2,       // a shift of build_soa_for_range out of the u32 range
     kErrNoTerminal = 3,  // row sequence passes endmark without a base frame
     kErrDepth = 4,       // terminal depth beyond kMaxDepth
@@ -114,7 +119,11 @@ NQ_HD inline uint32_t weight_shift(int ijkl, int N)
     return 3;
 }
 
-NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, uint32_t r_sij, Desc &D)
+// gate_bits: the rd word width build_soa's j-diagonal gate assumes (32, the
+// u32 kernels; the host tests lower it to put P5 records on small boards --
+// any width >= N counts the same).
+NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, uint32_t r_sij, Desc &D,
+                         int gate_bits = 32)
 {
     for (int a = 0; a < kMaxDepth / 4; a++) D.tab[a] = 0;
     D.status = kOk;
@@ -144,7 +153,9 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
         if (sh < 0) { D.status = kErrShift; return; }
         rd |= 1u << sh;
     }
-    if (j >= 2 * N - 33 - start) {
+    // j >= 2N-33-start at 32 bits: (N1-j) + (N2-start) < gate_bits
+    const bool j_gate = j >= 2 * N - 1 - gate_bits - start;
+    if (j_gate) {
         int sh = (N1 - j) + (N2 - start);
         if (N2 - start < 0 || sh > 31) { D.status = kErrShift; return; }
         rd |= 1u << sh;
@@ -157,7 +168,7 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     const bool l_eq_kp1 = l == k + 1, k_eq_lp1 = k == l + 1;
     if (j < N - 3) {
         jmark = j + 1; endmark = N2;
-        if (j > 2 * N - 34 - start) {
+        if (j_gate) {
             if (k_lt_l) {
                 mark1 = k - 1; mark2 = l - 1;
                 target = s_lt_l ? (s_lt_k ? (l_eq_kp1 ? 4 : 0) : 1) : 2;
@@ -209,12 +220,29 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     D.fid = target; D.jmark = jmark; D.endmark = endmark; D.mark1 = mark1; D.mark2 = mark2;
     D.wshift = weight_shift(ijkl, N);
     // ---- per-depth schedule: the functionid/row walk of dfs_iter ----
+    // P5 (functionids 8-11, the "SQBjl*" of
+    // 13Bit_codon/13Py_constellations_codon.py:620-678): records whose j
+    // diagonal failed the gate above start before that diagonal reaches the
+    // board.  At row N-1-jmark (= N-2-j) the frame takes it -- rd |= 1 << (N-1),
+    // column N-1 no longer free -- and continues as next_funcid on the same
+    // row; the rows above are placed without the one-row lookahead, as the
+    // SQBjl loops do.  18Py's dfs_iter has no such branch (its P5 frames never
+    // switch, 18Py:368-562), which is why its N=24 total is wrong (18Py:49).
+    // Here the bit comes from the byte of the switch frame's parent (JD), or
+    // is applied to the root when the root is the switch frame.
     int f = target, row = start, T = -1;
-    uint8_t jm[kMaxDepth + 1] = {};
     uint8_t qb[kMaxDepth] = {};
-    for (int d = 0; d < kMaxDepth; d++) {
+    bool root_jm = false, root_jd = false;
+    const int jd_row = N1 - jmark;
+    for (int d = 0; d < kMaxDepth;) {
         const int ptn = meta_ptn(f);
         if (ptn == 5 && row == endmark) { T = d; break; }
+        if (ptn == 4 && row == jd_row) {
+            if (d == 0) root_jd = true;
+            else qb[d - 1] |= 0x80u;
+            f = meta_next(f);
+            continue;   // the same frame, as next_funcid
+        }
         uint32_t s = 1, AL = 0, ARc = 0, LA = 0;
         if (ptn == 0 || ptn == 1 || ptn == 2) {
             const bool at = (ptn == 1) ? (row == mark2) : (row == mark1);
@@ -226,21 +254,21 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
             }
             // mark functionids have availptn 0: no lookahead on their plain rows
         } else if (ptn == 3 && row == jmark) {
-            jm[d] = 1;
-            AL = 2;
+            // this frame's ld |= 1: from its parent's AL bit 0, or on the root
+            if (d == 0) root_jm = true;
+            else qb[d - 1] |= 0x04u;
             LA = (meta_avail(f) && row + 1 < endmark) ? 1u : 0u;
             f = meta_next(f);
         } else if (ptn == 4) {
-            D.status = kErrP5; return;
+            if (row > jd_row) { D.status = kErrP5; return; }   // past its switch row
         } else {
             LA = (meta_avail(f) && row + 1 < endmark) ? 1u : 0u;
         }
         qb[d] = (uint8_t)(s | (AL << 2) | (ARc << 4) | ((LA ^ 1u) << 6));
         row += (int)s;
+        d++;
         if (row > N + 3) break;
     }
-    for (int d = 0; d < kMaxDepth; d++)
-        D.tab[d >> 2] |= (uint32_t)(qb[d] | (jm[d + 1] << 7)) << ((d & 3) * 8);
     if (T < 0) { D.status = kErrNoTerminal; return; }
     if (T >= kMaxDepth - 1) { D.status = kErrDepth; return; }
     D.T = (uint32_t)T;
@@ -248,9 +276,11 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
     // functionid 14 counts a terminal child only for columns other than 0:
     // the same as a jmark on the terminal row (which the row walk never
     // produces itself), so the walk needs no separate terminal mask
-    if (f == 14 && T > 0) D.tab[(T - 1) >> 2] |= 0x80u << (((T - 1) & 3) * 8);
-    // root frame, jmark applied when the root itself is the jmark frame
-    if (jm[0]) { fr &= ~1u; ld |= 1u; }
+    if (f == 14 && T > 0) qb[T - 1] |= 0x04u;
+    for (int d = 0; d < kMaxDepth; d++) D.tab[d >> 2] |= (uint32_t)qb[d] << ((d & 3) * 8);
+    // the root as the switch frame (its j diagonal) and as the jmark frame
+    if (root_jd) { fr &= ~(1u << N1); rd |= 1u << N1; }
+    if (root_jm) { fr &= ~1u; ld |= 1u; }
     D.ld = ld; D.rd = rd; D.col = col; D.avail = fr;
 }
 
@@ -264,7 +294,7 @@ NQ_HD inline void decode(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, ui
 // constellation (18Py:1749-1750).  emit(ld, rd, col, startijkl) receives the
 // children with columns ascending.  Returns kOk, kErrShift when the start row
 // is a pre-placed row or lies below the board, or kErrOverflow when a
-// child's field does not fit 32 bits (the record format is u32).
+// child's col or start row does not fit the u32 record.
 constexpr int32_t kErrOverflow = 6;
 template <class Emit>
 NQ_HD inline int32_t preset_children(int N, uint32_t r_ld, uint32_t r_rd, uint32_t r_col, uint32_t r_sij,
@@ -287,26 +317,27 @@ NQ_HD inline int32_t preset_children(int N, uint32_t r_ld, uint32_t r_rd, uint32
         const uint64_t ncol = (uint64_t)r_col | b;
         int nrow = row + 1;
         while (nrow == k || nrow == l) { nld <<= 1; nrd >>= 1; nrow++; }
-        if ((nld | ncol) >> 32 || nrow > 31) { status = kErrOverflow; continue; }
+        // ld bits at 32 and above are off the board (N <= 28 after the
+        // decode's ld >> 1) and only move further up: the u32 record drops them
+        if (ncol >> 32 || nrow > 31) { status = kErrOverflow; continue; }
         emit((uint32_t)nld, (uint32_t)nrd, (uint32_t)ncol, ((uint32_t)nrow << 20) | ijkl);
     }
     return status;
 }
 
 // The per-record schedule the kernels read (2 x 16 bytes):
-//   words 0-3   byte r (r = 1..15) = parameter byte of depth T-r: the byte a
-//               frame with r = T-1-depth gives its children; byte 0 (the
-//               children of a frame at depth T-1 are terminal) and bytes
-//               r >= T are zero
-//   word 4      T | wshift << 8 | N << 16 | parameter byte of depth 0 << 24
+//   words 0-3   byte r (r = 0..15) = parameter byte of depth T-1-r: the byte
+//               of a frame with r = T-1-depth (T <= 16, so depth 0 is byte
+//               T-1); bytes r >= T are zero
+//   word 4      T | wshift << 8 | N << 16
 //   word 5      terminal free mask (used for a root on the terminal row)
 //   word 6      decode status
 NQ_HD inline void sched_words(const Desc &D, int N, uint32_t w[8])
 {
     for (int a = 0; a < 8; a++) w[a] = 0;
-    for (uint32_t r = 1; r < 16u && r < D.T; r++)
-        w[r >> 2] |= param_byte(D.tab, D.T - r) << ((r & 3u) * 8u);
-    w[4] = D.T | (D.wshift << 8) | ((uint32_t)N << 16) | (param_byte(D.tab, 0) << 24);
+    for (uint32_t r = 0; r < 16u && r < D.T; r++)
+        w[r >> 2] |= param_byte(D.tab, D.T - 1u - r) << ((r & 3u) * 8u);
+    w[4] = D.T | (D.wshift << 8) | ((uint32_t)N << 16);
     w[5] = D.tm;
     w[6] = (uint32_t)D.status;
 }

@@ -1,5 +1,5 @@
 """python -m nqueens [nmin] [nmax] [--mode plain|batched] [--scheduler fifo|mlp] [--preset P] [--chunks K]
-                  [--chunk-range FIRST LAST] [--progress-file PATH]
+                  [--chunk-range FIRST LAST] [--progress-file PATH] [--cache-dir DIR]
 
 Prints the reference solver's table (13Bit_codon/18Py_constellations_GPU_cuda_codon.py:2115-2160)
 -- N, Total, Unique, hh:mm:ss.ms and ok/ng against A000170 (18Py:2089) --
@@ -9,7 +9,9 @@ the GPU.  With
 --chunks K each count runs as K device runs over interleaved slices of the
 constellations and prints a progress line after each; --chunk-range runs
 only some of them (a count split over several runs), --progress-file logs
-each counted chunk and skips the chunks the log already holds (resume).
+each counted chunk and skips the chunks the log already holds (resume),
+--cache-dir keeps the constellation list and per-constellation counts in the
+reference's cache files (nqueens.cache).
 """
 from __future__ import annotations
 
@@ -40,6 +42,9 @@ def main(argv=None) -> int:
     ap.add_argument("--chunk-range", type=int, nargs=2, metavar=("FIRST", "LAST"), default=None,
                     help="count only chunks FIRST..LAST-1 of --chunks and print their totals "
                          "(the totals of all chunks, from any runs, add up to the N-Queens total)")
+    ap.add_argument("--cache-dir", default=None,
+                    help="keep constellations_N{N}_{preset}.bin (+ .done) and solutions_N{N}_{preset}_v2.bin "
+                         "here, as the reference does: valid files are read, others rebuilt")
     ap.add_argument("--progress-file", default=None,
                     help="tab-separated per-chunk log (see nqueens.progress): chunks it already holds "
                          "are skipped, so a rerun resumes; the N-Queens row is printed once it is complete")
@@ -58,7 +63,8 @@ def main(argv=None) -> int:
                   f"{time.perf_counter() - t0:.1f} s", flush=True)
 
         total, unique = solver.count(N, preset=a.preset, mode=a.mode, scheduler=a.scheduler, ctx=ctx,
-                                     chunks=a.chunks, progress=progress if a.chunks > 1 else None)
+                                     chunks=a.chunks, progress=progress if a.chunks > 1 else None,
+                                     cache_dir=a.cache_dir, log=lambda m: print(f"  [cache] {m}", flush=True))
         dt = time.perf_counter() - t0
         ok = "ok" if EXPECTED.get(N) == total else f"ng({total}!={EXPECTED.get(N)})"
         bad += ok != "ok"

@@ -26,7 +26,7 @@ from ._lib import Context, Sched, as_records
 __all__ = ["count", "count_chunks", "count_tasks", "exec_solutions", "unique_count", "default_context", "default_sched",
            "preset_for", "constellations_for", "MIN_N", "MAX_N"]
 
-MIN_N, MAX_N = 6, 24      # the constellation path
+MIN_N, MAX_N = 6, 27      # the constellation path (the reference's range, 18Py:2089)
 SMALL_N = range(1, 6)     # plain full DFS, as 18Py's main does for N <= 5 (:2118-2126)
 
 _ctx: dict[int, Context] = {}
@@ -49,14 +49,14 @@ def preset_for(N: int) -> int:
     18Py always uses (:2139), up to N = 22.
 
     The walk keeps 16 parameter bytes per record, and a record of preset p
-    has its terminal row at depth N-1-p, so N = 23 needs preset 6 or more
-    and N = 24 preset 7 or more (where, unlike at preset 5, no record needs
-    the P5 functionids); a measured preset outside 4..7 or below N-17 is
-    not used."""
+    has its terminal row at depth N-1-p, so N = 23 needs preset 6 or more,
+    N = 24 preset 7 and N = 25 preset 8 (the reference's choice there,
+    359Py:5776-5785), N = 26 and 27 presets 9 and 10; a measured preset
+    outside 4..10 or below N-17 is not used."""
     from .autotune import tuned_preset
     lo = max(5, N - 17)
     p = tuned_preset(N)
-    if p is not None and max(4, N - 17) <= p <= 7:
+    if p is not None and max(4, N - 17) <= p <= 10:
         return p
     return lo
 
@@ -179,10 +179,27 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
     return totals
 
 
+def _count_list(ctx: Context, N: int, rec, mode: str, scheduler: str, sched: Sched | None, preset: int):
+    """Weighted count of every record of one list (one device run)."""
+    ctx.load(N, rec)
+    if scheduler == "mlp" and ctx.m:
+        from .costmodel import default_model
+        ctx.order_by_cost(default_model().params)
+    if mode == "plain":
+        ctx.run_plain()
+    else:
+        if sched is None:
+            from .autotune import best_sched
+            sched = best_sched(N, preset)
+        ctx.run_batched(sched)
+    w, _ = ctx.results(weighted=True)
+    return w
+
+
 def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler: str = "fifo",
           sched: Sched | None = None, ctx: Context | None = None, chunks: int = 1,
-          progress=None) -> tuple[int, int]:
-    """Total and unique N-Queens solutions for 1 <= N <= 24, counted on the GPU.
+          progress=None, cache_dir=None, log=None) -> tuple[int, int]:
+    """Total and unique N-Queens solutions for 1 <= N <= 27, counted on the GPU.
 
     N = 1..5 are counted by a plain full DFS over the board, as 18Py's main
     does for N <= 5 (_bit_total, :2071-2084, :2118-2126); N >= 6 through
@@ -190,13 +207,24 @@ def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler
     slices, one device run each (as the reference counts fixed-size chunks
     of its list, 359Py:5621-5626; see count_chunks), and calls
     progress(i, chunks, chunk_total) after each: a long count then reports
-    as it goes and no single launch holds the GPU for the whole run."""
+    as it goes and no single launch holds the GPU for the whole run.
+
+    cache_dir: keep the constellation list and the per-constellation counts
+    on disk in the reference's formats (nqueens.cache:
+    constellations_N{N}_{preset}.bin + .done, solutions_N{N}_{preset}_v2.bin,
+    18Py:1788-2068, 359Py:2960-3134): a valid list is read instead of
+    generated, a complete solutions file replaces the device count, and
+    whatever does not validate is built again and rewritten.  log(msg) hears
+    about rebuilt caches.  The list is counted in one run (chunks must be 1;
+    a chunked count resumes from a progress file instead)."""
     if chunks < 1:
         raise ValueError("chunks must be >= 1")
     if mode not in ("plain", "batched"):
         raise ValueError(f"unknown mode {mode!r}")
     if scheduler not in ("fifo", "mlp"):
         raise ValueError(f"unknown scheduler {scheduler!r}")
+    if cache_dir is not None and chunks != 1:
+        raise ValueError("cache_dir counts the whole list in one run (chunks must be 1)")
     if int(N) in SMALL_N:
         # one device launch; chunks, preset and schedule do not apply
         ctx = ctx or default_context()
@@ -212,7 +240,18 @@ def count(N: int, *, preset: int | None = None, mode: str = "batched", scheduler
         return total, unique_count(total, f90, f180)
     _check_n(N)
     ctx = ctx or default_context()
-    total = sum(count_chunks(N, chunks, preset=preset, mode=mode, scheduler=scheduler, sched=sched, ctx=ctx,
-                             progress=progress))
+    if cache_dir is not None:
+        from . import cache
+        preset = preset or preset_for(N)
+        rec, _ = cache.load_or_build_constellations(cache_dir, N, preset,
+                                                    lambda: constellations_for(N, preset, ctx), log)
+        w, _ = cache.load_or_build_solutions(cache_dir, N, preset, rec,
+                                             lambda r: _count_list(ctx, N, r, mode, scheduler, sched, preset), log)
+        total = int(w.sum(dtype=np.uint64))
+        if progress is not None:
+            progress(0, 1, total)
+    else:
+        total = sum(count_chunks(N, chunks, preset=preset, mode=mode, scheduler=scheduler, sched=sched, ctx=ctx,
+                                 progress=progress))
     f90, f180 = ctx.symmetric_counts(N)
     return total, unique_count(total, f90, f180)

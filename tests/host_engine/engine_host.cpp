@@ -32,7 +32,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     if (priv) {
         PrivStack st;
         st.init();
-        Walk w{ld, rd, start_col(S, col, d0), start_word(S, av, d0), st.empty()};
+        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
@@ -40,7 +40,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     } else {
         LdsStack st{m, 16u * block, 16u * (block + lane)};
         st.init();
-        Walk w{ld, rd, start_col(S, col, d0), start_word(S, av, d0), st.empty()};
+        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
@@ -65,7 +65,7 @@ namespace {
 // stacks get the layout the library allocates for the whole set
 // (walk_lds(max T, min item start depth)), else each item its own bound.
 int count_records(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes, int split,
-                  const uint8_t *depths, int priv, bool set_layout)
+                  const uint8_t *depths, int priv, bool set_layout, int gate_bits = 32)
 {
     WalkLds set_lay{0u};
     if (set_layout) {
@@ -73,7 +73,7 @@ int count_records(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64
         for (long a = 0; a < m; a++) {
             nqt::Desc D;
             const uint32_t *r = rec + 4 * a;
-            nqt::decode(N, r[0], r[1], r[2], r[3], D);
+            nqt::decode(N, r[0], r[1], r[2], r[3], D, gate_bits);
             if (D.status) return -1 - D.status;
             Tmax = D.T > Tmax ? D.T : Tmax;
             const uint32_t d = item_depth(depths ? depths[a] : (uint32_t)split, D.T);
@@ -85,7 +85,7 @@ int count_records(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64
         if (depths) split = depths[a];
         nqt::Desc D;
         const uint32_t *r = rec + 4 * a;
-        nqt::decode(N, r[0], r[1], r[2], r[3], D);
+        nqt::decode(N, r[0], r[1], r[2], r[3], D, gate_bits);
         if (D.status) return -1 - D.status;
         uint32_t wd[8];
         nqt::sched_words(D, N, wd);
@@ -125,6 +125,27 @@ extern "C" int engine_count(int N, const uint32_t *rec, long m, uint64_t *weight
                             int priv)
 {
     return count_records(N, rec, m, weighted, nodes, split, nullptr, priv, false);
+}
+
+// engine_count with build_soa's j-diagonal gate at gate_bits instead of 32
+// (records beyond it take the P5 functions; any width >= N counts the same).
+extern "C" int engine_count_gate(int N, const uint32_t *rec, long m, uint64_t *weighted, uint64_t *nodes,
+                                 int split, int priv, int gate_bits)
+{
+    return count_records(N, rec, m, weighted, nodes, split, nullptr, priv, false, gate_bits);
+}
+
+// P5 target count of the records under a gate width (decode fids 8..11).
+extern "C" long engine_p5_records(int N, const uint32_t *rec, long m, int gate_bits)
+{
+    long n = 0;
+    for (long a = 0; a < m; a++) {
+        nqt::Desc D;
+        const uint32_t *r = rec + 4 * a;
+        nqt::decode(N, r[0], r[1], r[2], r[3], D, gate_bits);
+        n += D.fid >= 8 && D.fid <= 11;
+    }
+    return n;
 }
 
 // The same with a split depth per record and the LDS layout sized for the

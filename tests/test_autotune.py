@@ -64,9 +64,9 @@ def test_record_presets_all_failed(tables):
 
 def test_preset_for_ignores_unusable_measurements(monkeypatch):
     # a table entry the walk cannot take (terminal row deeper than 16, or
-    # outside 4..7) falls back to the rule
+    # outside 4..10) falls back to the rule
     monkeypatch.setattr(autotune, "TUNED_PRESETS", {"23": {"preset": 5}, "12": {"preset": 3},
-                                                    "13": {"preset": 8}, "14": {"preset": 4}})
+                                                    "13": {"preset": 11}, "14": {"preset": 4}})
     assert solver.preset_for(23) == 6
     assert solver.preset_for(12) == 5
     assert solver.preset_for(13) == 5

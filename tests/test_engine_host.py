@@ -41,6 +41,11 @@ def engine(tmp_path_factory):
     lib.engine_count_splits.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u64p, u64p,
                                         np.ctypeslib.ndpointer(np.uint8, flags="C")]
     lib.engine_count_splits.restype = ctypes.c_int
+    lib.engine_count_gate.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u64p, u64p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
+    lib.engine_count_gate.restype = ctypes.c_int
+    lib.engine_p5_records.argtypes = [ctypes.c_int, u32p, ctypes.c_long, ctypes.c_int]
+    lib.engine_p5_records.restype = ctypes.c_long
     lib.engine_expand.argtypes = [ctypes.c_int, u32p, ctypes.c_long, u32p, ctypes.c_long]
     lib.engine_expand.restype = ctypes.c_long
     return lib
@@ -121,6 +126,30 @@ def test_engine_per_record_splits_set_layout(engine, oracle, N, lo, hi):
     rc = engine.engine_count_splits(N, np.ascontiguousarray(rec, np.uint32), len(rec), gw, gn, depth)
     assert rc == 0, rc
     assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+
+
+@pytest.mark.parametrize("N", [9, 11, 13, 15, 16])
+def test_engine_p5_records_under_narrow_gate(engine, oracle, N):
+    """With build_soa's j-diagonal gate narrowed from 32 bits to W (N <= W),
+    thousands of canonical records take the P5 functions (8..11) and insert
+    that diagonal at its switch row: the device engine's weighted counts and
+    node counts equal the oracle's under the same gate, at every split depth
+    and with both stacks, and the weighted counts equal the 32-bit ones."""
+    rec = np.ascontiguousarray(C.gen_constellations(N, 5), np.uint32)
+    w32, _, _ = oracle.count_records(N, rec)
+    for W in (N, N + 1, N + 3):
+        n5 = engine.engine_p5_records(N, rec, len(rec), W)
+        assert n5 > 0 or W > N
+        with oracle.gate_bits(W):
+            want, _, nodes = oracle.count_records(N, rec)
+        assert np.array_equal(want, w32)
+        for split, priv in ((0, 0), (2, 0), (3, 1)):
+            gw = np.zeros(len(rec), np.uint64)
+            gn = np.zeros(len(rec), np.uint64)
+            rc = engine.engine_count_gate(N, rec, len(rec), gw, gn, split, priv, W)
+            assert rc == 0, (W, split, rc)
+            assert np.array_equal(gw, want), (W, split)
+            assert np.array_equal(gn, nodes), (W, split)
 
 
 def _expand(engine, N, rec):

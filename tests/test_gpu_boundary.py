@@ -104,3 +104,34 @@ def test_small_boards_and_arguments(ctx):
     seen = []
     assert nqueens.count(5, ctx=ctx, progress=lambda i, k, t: seen.append((i, k, t))) == (10, 2)
     assert seen == [(0, 1, 10)]
+
+
+@pytest.mark.parametrize("N", range(12, 20))
+def test_count_through_reference_caches(ctx, digests, tmp_path, monkeypatch, N):
+    """count(N, cache_dir=...) writes the reference's cache files: the list
+    validates (.done = record count), the solutions v2 file read back holds
+    the reference fixture's per-constellation counts (SHA-256 of the sorted
+    stream), a second count is served from the files without a device count,
+    and a damaged solutions file is counted again and rewritten."""
+    from nqueens import cache, solver
+    from tests.helpers import solutions_sha256
+    known_total = digests[str(N)]["solutions_sum"]
+    msgs = []
+    total, unique = nqueens.count(N, preset=5, ctx=ctx, cache_dir=tmp_path, log=msgs.append)
+    assert total == known_total and msgs == []
+    rec = cache.load_constellations(tmp_path, N, 5)
+    assert rec is not None and rec.shape[0] == digests[str(N)]["constellations"]
+    frec, fsol = nqueens.constellations.read_solutions_bin(cache.solutions_path(tmp_path, N, 5))
+    assert solutions_sha256(frec, fsol) == digests[str(N)]["solutions_sha256"]
+
+    def no_device_count(*a, **k):
+        raise AssertionError("the cache should have answered")
+    with monkeypatch.context() as mp:
+        mp.setattr(solver, "_count_list", no_device_count)
+        assert nqueens.count(N, preset=5, ctx=ctx, cache_dir=tmp_path) == (total, unique)
+    p = cache.solutions_path(tmp_path, N, 5)
+    p.write_bytes(p.read_bytes()[:-40])
+    assert nqueens.count(N, preset=5, ctx=ctx, cache_dir=tmp_path, log=msgs.append) == (total, unique)
+    assert len(msgs) == 1 and "counting again" in msgs[0]
+    frec, fsol = nqueens.constellations.read_solutions_bin(p)
+    assert solutions_sha256(frec, fsol) == digests[str(N)]["solutions_sha256"]

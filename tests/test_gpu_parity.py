@@ -137,10 +137,11 @@ def test_empty_set_and_errors(ctx):
     assert ctx.run_plain()[0] == 0
     assert ctx.run_batched(sched())[0] == 0
     with pytest.raises(nqueens.NQError):
-        ctx.load(25, C.gen_constellations(12, 5))
-    # a P5 target (functionid 8..11) is refused, never counted
+        ctx.load(29, C.gen_constellations(12, 5))
+    # a record the decode cannot place (its row walk never reaches a base
+    # frame) is refused, never counted
     bad = np.array([[0, 0, 0, (2 << 20) | C.to_ijkl(5, 3, 7, 9)]], np.uint32)
-    assert nqueens.decode_host(21, bad)[0, 9] == 1
+    assert nqueens.decode_host(21, bad)[0, 9] == 3
     with pytest.raises(nqueens.NQError) as e:
         ctx.load(21, bad)
     assert e.value.code == -2
@@ -406,3 +407,75 @@ def test_small_boards(ctx, known, oracle):
         assert ctx.small_total(N) == nqueens.count(N, ctx=ctx)[0] == known["total_a000170"][str(N)]
     with pytest.raises(nqueens.NQError):
         ctx.small_total(13)
+
+
+@pytest.fixture(scope="module")
+def p5_n24():
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "p5_n24.npz") as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_p5_records_of_n24(ctx, p5_n24, mode):
+    """The 18 P5 records of N=24 preset 5 (functionids 8..11, the j diagonal
+    set on its switch row): weighted and node counts equal the oracle's P5
+    branch record by record, and each record's count equals the count of its
+    descendants two preset queens deeper, expanded on the device (no P5 among
+    them)."""
+    rec = p5_n24["rec"]
+    d = nqueens.decode_host(24, rec)
+    assert set(d[:, 0].tolist()) <= {8, 9, 10, 11} and (d[:, 9] == 0).all()
+    ctx.load(24, rec)
+    if mode == "plain":
+        total, _ = ctx.run_plain(count_nodes=True)
+    else:
+        total, _ = ctx.run_batched(sched(3, 256, 1), count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, p5_n24["w"]) and np.array_equal(gn, p5_n24["nodes"])
+    kids = [ctx.expand_records(24, rec[a:a + 1], 2) for a in range(rec.shape[0])]
+    assert [k.shape[0] for k in kids] == p5_n24["kids"].tolist()
+    allk = np.concatenate(kids)
+    assert not np.isin(nqueens.decode_host(24, allk)[:, 0], [8, 9, 10, 11]).any()
+    ctx.load(24, allk)
+    ctx.run_batched(sched(3, 256, 1), count_nodes=True)
+    kw, kn = ctx.results(weighted=True, nodes=True)
+    b = np.cumsum([0] + [k.shape[0] for k in kids])
+    assert [int(kw[b[a]:b[a + 1]].sum()) for a in range(len(kids))] == gw.tolist()
+    assert [int(kn[b[a]:b[a + 1]].sum()) for a in range(len(kids))] == p5_n24["kids_nodes"].tolist()
+
+
+def test_n25_preset8_oracle_sample(ctx):
+    """N=25 at preset 8, the reference's preset there (terminal depth 16, a
+    25-bit board): the device expansion of every 20,000th preset-5 record by
+    three preset queens gives the oracle's records (sampled every 448th), and
+    their weighted and node counts equal the oracle's, batched and plain."""
+    from tests.conftest import GOLDEN
+    with np.load(GOLDEN / "oracle_sample_n25.npz") as z:
+        f = {k: z[k] for k in z.files}
+    parents = C.gen_constellations(25, 5)[::20000]
+    assert np.array_equal(parents, f["parents"])
+    kids = ctx.expand_records(25, parents, 3)
+    assert np.array_equal(kids[::448], f["rec25"])
+    ctx.load(25, f["rec25"])
+    for mode in ("batched", "plain"):
+        if mode == "plain":
+            total, _ = ctx.run_plain(count_nodes=True)
+        else:
+            total, _ = ctx.run_batched(sched(3, 256, 1), count_nodes=True)
+        gw, gn = ctx.results(weighted=True, nodes=True)
+        assert np.array_equal(gw, f["w25"]) and np.array_equal(gn, f["nodes25"]), mode
+        assert total == int(f["w25"].sum())
+
+
+def test_symmetric_counts_n25_n27(ctx, known):
+    """F90/F180 past N=24: at N=25 they give the published Unique through
+    Burnside; at N=26 and 27 (no published Unique in the reference) the orbit
+    count is integral."""
+    from nqueens.expected import TOTAL, UNIQUE
+    f90, f180 = ctx.symmetric_counts(25)
+    assert TOTAL[25] + f180 + 2 * f90 == 8 * UNIQUE[25]
+    for N in (26, 27):
+        f90, f180 = ctx.symmetric_counts(N)
+        assert (TOTAL[N] + f180 + 2 * f90) % 8 == 0 and f180 >= f90 >= 0
+        print(f"N={N}: F90={f90} F180={f180} unique={(TOTAL[N] + f180 + 2 * f90) // 8}", flush=True)

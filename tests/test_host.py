@@ -89,7 +89,7 @@ def test_decode_matches_oracle_build_soa(oracle, N, preset):
 def test_decode_rejects_out_of_range():
     rec = C.gen_constellations(12, 5)
     with pytest.raises(nqueens.NQError):
-        nqueens.decode_host(25, rec)
+        nqueens.decode_host(29, rec)
     bad = rec.copy()
     bad[:, 3] = (31 << 20) | (bad[:, 3] & 0xFFFFF)   # start row beyond the board
     d = nqueens.decode_host(12, bad)
@@ -124,7 +124,7 @@ def test_solver_argument_checks():
     with pytest.raises(ValueError):
         nqueens.count(0)      # below the small-board range (1..5)
     with pytest.raises(ValueError):
-        nqueens.count(25)
+        nqueens.count(28)
     # checked before the small-board branch too (no device work is started)
     for kw in ({"chunks": 0}, {"mode": "cpu"}, {"scheduler": "lifo"}):
         for n in (1, 4, 12):
@@ -135,14 +135,14 @@ def test_solver_argument_checks():
     assert nqueens.unique_count(14200, 8, 80) == 1787
     # chunk ranges are checked before any device work
     from nqueens.solver import count_chunks, preset_for
-    for args in ((16, 0), (16, 4, 3, 2), (16, 4, 0, 5), (16, 4, -1, 2), (25, 4)):
+    for args in ((16, 0), (16, 4, 3, 2), (16, 4, 0, 5), (16, 4, -1, 2), (28, 4)):
         with pytest.raises(ValueError):
             count_chunks(*args)
     with pytest.raises(ValueError):
         nqueens.count(16, chunks=0)
     # terminal depth N-1-preset stays within the walk's 16 parameter bytes
-    assert [preset_for(n) for n in (22, 23, 24)] == [5, 6, 7]    # no measured preset: the rule
-    assert all(n - 1 - preset_for(n) <= 16 for n in range(6, 25))
+    assert [preset_for(n) for n in (22, 23, 24, 25, 26, 27)] == [5, 6, 7, 8, 9, 10]   # no measured preset: the rule
+    assert all(n - 1 - preset_for(n) <= 16 for n in range(6, 28))
 
 
 def test_expected_covers_every_countable_board():
