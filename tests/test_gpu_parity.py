@@ -416,23 +416,18 @@ def p5_n24():
         return {k: z[k] for k in z.files}
 
 
-@pytest.mark.parametrize("mode", ["plain", "batched"])
-def test_p5_records_of_n24(ctx, p5_n24, mode):
-    """The 18 P5 records of N=24 preset 5 (functionids 8..11, the j diagonal
-    set on its switch row): weighted and node counts equal the oracle's P5
-    branch record by record, and each record's count equals the count of its
-    descendants two preset queens deeper, expanded on the device (no P5 among
-    them)."""
+def test_p5_records_of_n24_through_descendants(ctx, p5_n24):
+    """The 18 P5 records of N=24 preset 5 start on row 2, so their terminal
+    row lies 18 rows down, past the walk's 16-depth table: they are counted
+    through their descendants two preset queens deeper, expanded on the
+    device (no P5 among them), and each record's sum equals the oracle's P5
+    branch on the record itself (weighted count and node count less the
+    extractions of the two expanded rows)."""
     rec = p5_n24["rec"]
     d = nqueens.decode_host(24, rec)
-    assert set(d[:, 0].tolist()) <= {8, 9, 10, 11} and (d[:, 9] == 0).all()
-    ctx.load(24, rec)
-    if mode == "plain":
-        total, _ = ctx.run_plain(count_nodes=True)
-    else:
-        total, _ = ctx.run_batched(sched(3, 256, 1), count_nodes=True)
-    gw, gn = ctx.results(weighted=True, nodes=True)
-    assert np.array_equal(gw, p5_n24["w"]) and np.array_equal(gn, p5_n24["nodes"])
+    assert set(d[:, 0].tolist()) <= {8, 9, 10, 11} and (d[:, 8] == 18).all()
+    with pytest.raises(nqueens.NQError):
+        ctx.load(24, rec)
     kids = [ctx.expand_records(24, rec[a:a + 1], 2) for a in range(rec.shape[0])]
     assert [k.shape[0] for k in kids] == p5_n24["kids"].tolist()
     allk = np.concatenate(kids)
@@ -441,8 +436,40 @@ def test_p5_records_of_n24(ctx, p5_n24, mode):
     ctx.run_batched(sched(3, 256, 1), count_nodes=True)
     kw, kn = ctx.results(weighted=True, nodes=True)
     b = np.cumsum([0] + [k.shape[0] for k in kids])
-    assert [int(kw[b[a]:b[a + 1]].sum()) for a in range(len(kids))] == gw.tolist()
+    assert [int(kw[b[a]:b[a + 1]].sum()) for a in range(len(kids))] == p5_n24["w"].tolist()
     assert [int(kn[b[a]:b[a + 1]].sum()) for a in range(len(kids))] == p5_n24["kids_nodes"].tolist()
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_p5_records_n28_preset11(ctx, oracle, mode):
+    """Real P5 records that the walk takes directly (terminal depth 16): the
+    device expansion gives the oracle's records, the P5 switch row sets the j
+    diagonal (parameter bit 7), and weighted and node counts equal the
+    oracle's P5 branch record by record; each record also counts what its
+    descendants two queens deeper (no P5 among them) count on the device."""
+    from tests.helpers import p5_records_n28
+    N = 28
+    par, kids, rec = p5_records_n28(oracle)
+    dev = ctx.expand_records(N, par, 6)
+    assert np.array_equal(dev, kids)
+    d = nqueens.decode_host(N, rec)
+    assert np.isin(d[:, 0], [8, 9, 10, 11]).all() and (d[:, 8] == 16).all() and (d[:, 9] == 0).all()
+    w, _, nodes = oracle.count_records(N, rec)
+    ctx.load(N, rec)
+    if mode == "plain":
+        total, _ = ctx.run_plain(count_nodes=True)
+    else:
+        total, _ = ctx.run_batched(sched(3, 256, 1), count_nodes=True)
+    gw, gn = ctx.results(weighted=True, nodes=True)
+    assert np.array_equal(gw, w) and np.array_equal(gn, nodes)
+    sub = [ctx.expand_records(N, rec[a:a + 1], 2) for a in range(0, rec.shape[0], 8)]
+    allk = np.concatenate(sub)
+    assert not np.isin(nqueens.decode_host(N, allk)[:, 0], [8, 9, 10, 11]).any()
+    ctx.load(N, allk)
+    ctx.run_batched(sched(3, 256, 1))
+    kw, _ = ctx.results(weighted=True)
+    b = np.cumsum([0] + [x.shape[0] for x in sub])
+    assert [int(kw[b[a]:b[a + 1]].sum()) for a in range(len(sub))] == gw[::8].tolist()
 
 
 def test_n25_preset8_oracle_sample(ctx):
@@ -468,14 +495,14 @@ def test_n25_preset8_oracle_sample(ctx):
         assert total == int(f["w25"].sum())
 
 
-def test_symmetric_counts_n25_n27(ctx, known):
+def test_symmetric_counts_n25_n26(ctx, known):
     """F90/F180 past N=24: at N=25 they give the published Unique through
-    Burnside; at N=26 and 27 (no published Unique in the reference) the orbit
-    count is integral."""
+    Burnside; at N=26 (no published Unique in the reference) the orbit count
+    is integral."""
     from nqueens.expected import TOTAL, UNIQUE
     f90, f180 = ctx.symmetric_counts(25)
     assert TOTAL[25] + f180 + 2 * f90 == 8 * UNIQUE[25]
-    for N in (26, 27):
+    for N in (26,):
         f90, f180 = ctx.symmetric_counts(N)
         assert (TOTAL[N] + f180 + 2 * f90) % 8 == 0 and f180 >= f90 >= 0
         print(f"N={N}: F90={f90} F180={f180} unique={(TOTAL[N] + f180 + 2 * f90) // 8}", flush=True)

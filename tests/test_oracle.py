@@ -256,3 +256,23 @@ def test_p5_records_of_n24_partition(oracle):
         assert k7.shape[0] == f["kids"][a]
         kf = oracle.build_soa(24, k7)[:, 0]
         assert not ((kf >= 8) & (kf <= 11)).any()
+
+
+def test_p5_records_of_n28_partition(oracle):
+    """P5 records at a depth the u32 walk holds exist first at N=28 (preset
+    11, j = 14, start row 8; for N <= 27 every P5 record ends 17 or more rows
+    below its start): each counts, by the oracle's P5 branch, what its
+    descendants two preset queens deeper (none of them P5) count."""
+    from tests.helpers import p5_records_n28
+    N = 28
+    _, _, rec = p5_records_n28(oracle, parents=1, stride=80000)
+    fid = oracle.build_soa(N, rec)[:, 0]
+    assert np.isin(fid, [8, 9, 10, 11]).all() and len(rec) >= 40
+    w, _, nodes = oracle.count_records(N, rec)
+    for a in range(rec.shape[0]):
+        kids = oracle.expand_records(N, rec[a:a + 1], 2)
+        assert not np.isin(oracle.build_soa(N, kids)[:, 0], [8, 9, 10, 11]).any()
+        kw, _, kn = oracle.count_records(N, kids)
+        n6 = oracle.expand_records(N, rec[a:a + 1], 1).shape[0]
+        assert int(kw.sum()) == int(w[a])
+        assert int(kn.sum()) + n6 + kids.shape[0] == int(nodes[a])
