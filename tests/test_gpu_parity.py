@@ -495,14 +495,14 @@ def test_n25_preset8_oracle_sample(ctx):
         assert total == int(f["w25"].sum())
 
 
-def test_symmetric_counts_n25_n26(ctx, known):
+def test_symmetric_counts_n25_n27(ctx, known):
     """F90/F180 past N=24: at N=25 they give the published Unique through
-    Burnside; at N=26 (no published Unique in the reference) the orbit count
-    is integral."""
+    Burnside; at N=26 and 27 (no published Unique in the reference) the
+    orbit count is integral (seconds each)."""
     from nqueens.expected import TOTAL, UNIQUE
     f90, f180 = ctx.symmetric_counts(25)
     assert TOTAL[25] + f180 + 2 * f90 == 8 * UNIQUE[25]
-    for N in (26,):
+    for N in (26, 27):
         f90, f180 = ctx.symmetric_counts(N)
         assert (TOTAL[N] + f180 + 2 * f90) % 8 == 0 and f180 >= f90 >= 0
         print(f"N={N}: F90={f90} F180={f180} unique={(TOTAL[N] + f180 + 2 * f90) // 8}", flush=True)
