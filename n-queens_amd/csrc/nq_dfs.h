@@ -212,11 +212,10 @@ struct Walk {
 // with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots
 // above the inert frame's.  Returns true when the walk's subtree is
 // exhausted (the step took the inert frame).
-// ISSUE: spend two full-rate VALU ops where one slower op would do (the
-// child's frame word).  That pays where the SIMD is issue-bound (the
-// persistent kernel, 16 waves per CU) and costs where a lane's dependency
-// chain is the limit (one lane per constellation).
-template <bool NODES, bool ISSUE = true, class Count, class Stack>
+// This is the step's definition: the host engine and the one-lane-per-
+// constellation kernel run it as written; the persistent kernel runs the
+// same step with a fixed instruction order (walk_step_paired below).
+template <bool NODES, class Count, class Stack>
 NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint64_t &nodes)
 {
     const uint32_t S = st.S;
@@ -262,11 +261,7 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
     const uint32_t A = w.A;
     // the child: r - 1, candidates nf
-    uint32_t Wc = hiW + nf - (1u << kRShift);
-#if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (ISSUE)   // two two-operand adds, not a v_add3 with a scalar operand
-        asm("v_add_u32 %0, %1, %2" : "=v"(Wc) : "v"(hiW - (1u << kRShift)), "v"(nf));
-#endif
+    const uint32_t Wc = hiW + nf - (1u << kRShift);
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
     w.col = down ? ncol : (up ? top.z : C);
@@ -279,6 +274,127 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 #endif
     if (push) st.write(A + S, rem);
     return up & !nonempty;
+}
+
+// ---------------------------------------------------------------------------
+// walk_step with a fixed instruction order (the persistent kernel's step)
+// ---------------------------------------------------------------------------
+// On gfx950, with four waves per SIMD, the simple 32-bit ops (add, sub, and,
+// or, xor, right shifts, bitop3, mov, cndmask, add/sub with carry) issue in
+// about 2.6 clocks per wave instruction when their neighbours are simple ops
+// too; v_perm, v_bfe, v_lshlrev, every v_cmp, v_min/v_max, v_mul_u24,
+// v_bfi and the three-input adds take about 4.3, and placed one at a time
+// between simple ops about twice that.  The compiler's schedule of
+// walk_step scatters eleven such ops over the step.  This form keeps three
+// of them, in one dependent cluster (perm -> bfe -> lshl), and writes every
+// condition as an add/sub carry-out into an SGPR pair (x != 0: the carry of
+// x + 0xFFFFFFFF; b > a: the borrow of a - b), which the scalar unit then
+// combines.  The parameter byte comes from one perm over the two table words
+// selected by W's sign (r >= 8), with selector byte 3 = r & 7.
+// The step is two asm blocks: the body, and the selects, which the compiler
+// places after its wait for the top entry; one block per step would need
+// the frame in a register quadruple for the stack write.  Operands and
+// results are exactly walk_step's.
+struct PairK {
+    uint32_t hi;      // 0xF0000000: the r field
+    uint32_t S, nS;   // the stack's slot stride and its negative
+    uint32_t sel;     // 0x07FFFFFF: keeps r & 7 in byte 3 of the perm selector
+};
+#if defined(__HIP_DEVICE_COMPILE__)
+template <bool NODES, class Stack>
+__device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w, Stack &st, uint32_t &cnt,
+                                        uint64_t &nodes)
+{
+    const uint4 top = st.read(w.A);
+    const uint32_t W = w.W, ld = w.ld, rd = w.rd, C = w.col, A = w.A, E = st.empty();
+    uint32_t nld, nrd, ncol, rest, Wc, ta, tb, tc, td, te, tf;
+    uint64_t m1, m2, m3, up, down, push, pop;
+    asm volatile(
+        // byte lookup and the ld shift: the one cluster of slow ops
+        "v_ashrrev_i32 %[c], 31, %[W]\n\t"                        // r >= 8
+        "v_lshrrev_b32 %[a], 4, %[W]\n\t"
+        "v_bitop3_b32 %[b], %[c], %[t2], %[t0] bitop3:0xca\n\t"  // table words of r's half
+        "v_bitop3_b32 %[e], %[c], %[t3], %[t1] bitop3:0xca\n\t"
+        "v_and_b32 %[a], %[a], %[sel]\n\t"                       // selector, byte 3 = r & 7
+        "v_sub_u32 %[d], 0, %[W]\n\t"
+        "v_and_b32 %[d], %[W], %[d]\n\t"                         // bit
+        "v_or_b32 %[f], %[ld], %[d]\n\t"
+        "v_perm_b32 %[c], %[e], %[b], %[a]\n\t"                  // q: the byte in bits 24-31
+        "v_bfe_u32 %[b], %[c], 24, 2\n\t"                        // s
+        "v_lshlrev_b32 %[a], %[b], %[f]\n\t"                     // x = (ld | bit) << s
+        // the child
+        "v_or_b32 %[e], %[rd], %[d]\n\t"
+        "v_lshrrev_b32 %[e], %[b], %[e]\n\t"
+        "v_lshrrev_b32 %[b], 26, %[c]\n\t"
+        "v_bitop3_b32 %[nld], %[a], %[b], %[three] bitop3:0xf8\n\t"
+        "v_lshrrev_b32 %[b], %[ksh], %[c]\n\t"
+        "v_bitop3_b32 %[nrd], %[e], %[b], %[km] bitop3:0xf8\n\t"
+        "v_or_b32 %[ncol], %[col], %[d]\n\t"
+        "v_xor_b32 %[rest], %[W], %[d]\n\t"
+        // the one-row lookahead and the child's free set
+        "v_add_u32 %[a], %[a], %[a]\n\t"
+        "v_lshrrev_b32 %[b], 1, %[nrd]\n\t"
+        "v_bitop3_b32 %[a], %[a], %[b], %[ncol] bitop3:0xfe\n\t"
+        "v_and_b32 %[c], %[c], %[nlab]\n\t"
+        "v_bitop3_b32 %[a], %[mask], %[a], %[c] bitop3:0xba\n\t"  // lookahead passes (or none here)
+        "v_or_b32 %[b], %[nld], %[nrd]\n\t"
+        "v_bitop3_b32 %[b], %[mask], %[b], %[ncol] bitop3:0x10\n\t"  // nf
+        "v_and_b32 %[d], %[W], %[hi]\n\t"                        // r << 28
+        "v_add_u32 %[e], %[d], %[hi]\n\t"
+        "v_add_u32 %[Wc], %[e], %[b]\n\t"                        // the child's word
+        // conditions
+        "v_add_co_u32_e64 %[e], %[m1], %[b], -1\n\t"             // nf != 0
+        "v_add_co_u32_e64 %[e], %[m2], %[a], -1\n\t"             // lookahead
+        "v_add_co_u32_e64 %[e], %[m3], %[d], -1\n\t"             // inner: r != 0
+        "s_and_b64 %[m1], %[m1], %[m2]\n\t"                      // ok
+        "v_sub_co_u32_e64 %[e], %[m2], %[d], %[rest]\n\t"        // more: rest > r << 28
+        "s_and_b64 %[down], %[m1], %[m3]\n\t"
+        "s_andn2_b64 %[m1], %[m1], %[m3]\n\t"                    // sol
+        "v_sub_co_u32_e64 %[e], %[m3], %[E], %[A]\n\t"           // nonempty: A > E
+        "s_and_b64 %[push], %[down], %[m2]\n\t"
+        "s_nor_b64 %[up], %[down], %[m2]\n\t"
+        "s_and_b64 %[pop], %[up], %[m3]\n\t"
+        "v_addc_co_u32_e64 %[cnt], %[m2], 0, %[cnt], %[m1]"     // cnt += sol
+        : [nld] "=&v"(nld), [nrd] "=&v"(nrd), [ncol] "=&v"(ncol), [rest] "=&v"(rest), [Wc] "=&v"(Wc),
+          [a] "=&v"(ta), [b] "=&v"(tb), [c] "=&v"(tc), [d] "=&v"(td), [e] "=&v"(te), [f] "=&v"(tf),
+          [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [up] "=&s"(up), [down] "=&s"(down), [push] "=&s"(push),
+          [pop] "=&s"(pop), [cnt] "+v"(cnt)
+        : [W] "v"(W), [ld] "v"(ld), [rd] "v"(rd), [col] "v"(C), [A] "v"(A), [E] "v"(E), [t0] "v"(L.t0),
+          [t1] "v"(L.t1), [t2] "v"(L.t2), [t3] "v"(L.t3), [ksh] "v"(L.ksh), [km] "v"(L.km), [three] "v"(L.three),
+          [mask] "v"(L.mask), [nlab] "v"(L.nlab), [hi] "v"(K.hi), [sel] "v"(K.sel)
+        : "scc");
+    if (NODES) nodes += (uint64_t)(W != 0u);             // the inert frame is no node
+    uint32_t l1, r1, c1, w1, dA;
+    asm volatile(
+        "v_cndmask_b32_e64 %[dA], 0, %[nS], %[pop]\n\t"
+        "v_cndmask_b32_e64 %[l1], %[ld], %[tx], %[up]\n\t"
+        "v_cndmask_b32_e64 %[dA], %[dA], %[S], %[push]\n\t"
+        "v_cndmask_b32_e64 %[r1], %[rd], %[ty], %[up]\n\t"
+        "v_cndmask_b32_e64 %[lo], %[l1], %[nld], %[down]\n\t"
+        "v_cndmask_b32_e64 %[c1], %[col], %[tz], %[up]\n\t"
+        "v_cndmask_b32_e64 %[ro], %[r1], %[nrd], %[down]\n\t"
+        "v_cndmask_b32_e64 %[w1], %[rest], %[tw], %[up]\n\t"
+        "v_cndmask_b32_e64 %[co], %[c1], %[ncol], %[down]\n\t"
+        "v_add_u32 %[Ao], %[A], %[dA]\n\t"
+        "v_cndmask_b32_e64 %[Wo], %[w1], %[Wc], %[down]"
+        : [lo] "=&v"(w.ld), [ro] "=&v"(w.rd), [co] "=&v"(w.col), [Wo] "=&v"(w.W), [Ao] "=&v"(w.A),
+          [l1] "=&v"(l1), [r1] "=&v"(r1), [c1] "=&v"(c1), [w1] "=&v"(w1), [dA] "=&v"(dA)
+        : [up] "s"(up), [down] "s"(down), [push] "s"(push), [pop] "s"(pop), [tx] "v"(top.x), [ty] "v"(top.y),
+          [tz] "v"(top.z), [tw] "v"(top.w), [ld] "v"(ld), [rd] "v"(rd), [col] "v"(C), [rest] "v"(rest),
+          [nld] "v"(nld), [nrd] "v"(nrd), [ncol] "v"(ncol), [Wc] "v"(Wc), [A] "v"(A), [S] "v"(K.S), [nS] "v"(K.nS));
+    if (__builtin_amdgcn_inverse_ballot_w64(push)) st.write(A + st.S, make_uint4(ld, rd, C, rest));
+}
+#else
+// host pass of the kernels only (kernels never run on the host)
+template <bool NODES, class Stack>
+NQ_HD inline void walk_step_paired(const LaneK &L, const PairK &, Walk &w, Stack &st, uint32_t &cnt, uint64_t &nodes)
+{
+    (void)walk_step<NODES>(L, w, st, cnt, nodes);
+}
+#endif
+inline NQ_HD PairK pair_consts(uint32_t S)
+{
+    return PairK{vconst(0xF0000000u), vconst(S), vconst(0u - S), vconst(0x07FFFFFFu)};
 }
 
 // ---------------------------------------------------------------------------

@@ -34,7 +34,7 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
     Walk w{root.x, root.y, root.z, start_word(S, root.w, 0u), st.empty()};
-    while (!walk_step<NODES, false>(L, w, st, cnt, nodes)) {   // one lane per record: latency-bound
+    while (!walk_step<NODES>(L, w, st, cnt, nodes)) {   // one lane per record: latency-bound
     }
 }
 
@@ -277,9 +277,10 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
     uint4 p0{}, p1{}, p2{};
     uint32_t c = 0, cnt = 0, fc = 0, fcnt = 0;
     uint64_t nn = 0, fnn = 0;
+    const PairK K = pair_consts(st.S);
     for (;;) {
 #pragma unroll
-        for (int u = 0; u < kStepsPerCheck; u++) (void)walk_step<NODES>(L, w, st, cnt, nn);
+        for (int u = 0; u < kStepsPerCheck; u++) walk_step_paired<NODES>(L, K, w, st, cnt, nn);
         const bool inert = w.W == 0u;   // a walking frame always has candidates
         if (wave_ballot(inert || cnt >= kCountFlush) == 0ull) continue;
         if (have && inert) {
