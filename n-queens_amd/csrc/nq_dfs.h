@@ -212,9 +212,9 @@ struct Walk {
 // with remaining candidates are pushed from depths d0..T-2: T-1-d0 slots
 // above the inert frame's.  Returns true when the walk's subtree is
 // exhausted (the step took the inert frame).
-// This is the step's definition: the host engine and the one-lane-per-
-// constellation kernel run it as written; the persistent kernel runs the
-// same step with a fixed instruction order (walk_step_paired below).
+// This is the step's definition, which the host engine runs as written; the
+// kernels run the same step with a fixed instruction order
+// (walk_step_paired below).
 template <bool NODES, class Count, class Stack>
 NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint64_t &nodes)
 {
@@ -277,7 +277,7 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 }
 
 // ---------------------------------------------------------------------------
-// walk_step with a fixed instruction order (the persistent kernel's step)
+// walk_step with a fixed instruction order (the kernels' step)
 // ---------------------------------------------------------------------------
 // On gfx950, with four waves per SIMD, the simple 32-bit ops (add, sub, and,
 // or, xor, right shifts, bitop3, mov, cndmask, add/sub with carry) issue in

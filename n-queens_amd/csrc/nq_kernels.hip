@@ -25,7 +25,17 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Count + nodes of a whole constellation from its root frame (depth 0).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Steps between two looks at the wave's lanes: a lane whose walk ends sits
+// on the inert frame and waits at most this many steps (for its next item,
+// or for the wave's other lanes); the steps in between carry no wave-wide
+// check.
+constexpr int kStepsPerCheck = 32;
+
+// Count + nodes of a whole constellation from its root frame (depth 0).  A
+// lane whose walk has ended steps on the inert frame until every walk of its
+// wave has ended.
 template <bool NODES, class Stack>
 __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, uint64_t &cnt, uint64_t &nodes)
 {
@@ -33,12 +43,15 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (root.w == 0u) return;
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
+    const PairK K = pair_consts(st.S);
     Walk w{root.x, root.y, root.z, start_word(S, root.w, 0u), st.empty()};
-    while (!walk_step<NODES>(L, w, st, cnt, nodes)) {   // one lane per record: latency-bound
-    }
+    do {
+        uint32_t c32 = 0;
+#pragma unroll
+        for (int u = 0; u < kStepsPerCheck; u++) walk_step_paired<NODES>(L, K, w, st, c32, nodes);
+        cnt += c32;
+    } while (wave_ballot(w.W != 0u) != 0ull);
 }
-
-__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // ---------------------------------------------------------------------------
 // decode: records -> root frames + schedule descriptors
@@ -244,10 +257,6 @@ template <> struct BatchedStack<false> { using type = PrivStack; };
 // step adds at most one, so it never wraps).
 constexpr uint32_t kCountFlush = 1u << 31;
 
-// Steps between two looks at the wave's lanes: a lane whose walk ends sits
-// on the inert frame and waits at most this many steps for its next item;
-// the steps in between carry no wave-wide check.
-constexpr int kStepsPerCheck = 32;
 
 // Each lane walks one item and holds the next one, loaded while it walks:
 // when its walk ends the lane switches to the held item without waiting for
