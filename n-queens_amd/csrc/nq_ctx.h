@@ -69,6 +69,7 @@ struct nq_ctx {
     int N = 0;                    // board size of the set, 0 for a mixed set
     uint64_t m = 0;
     uint32_t Tmax = 0, Tmin = 0;
+    uint32_t fan = 1;             // expansion lanes per record: the set's largest N
     bool loaded = false;
     nqk_buf::Buf<uint4> rec, root, sched, items;
     nqk_buf::Buf<uint32_t> order, nitems, off, err, head;
@@ -78,7 +79,7 @@ struct nq_ctx {
     uint32_t split_min = 0;       // shallowest depth a per-record split starts an item at
     std::vector<uint8_t> Trec;    // terminal depth per record (host copy, from the decode)
     nqk_buf::Buf<unsigned long long> weighted, nodes, raw, total, item_count, sym;
-    nqk_buf::Buf<unsigned long long> scan_tmp;   // tile sums of the item-offset scan, (m + 1023) / 1024
+    nqk_buf::Buf<unsigned long long> scan_tmp;   // tile sums of the item-offset scan, (m * fan + 1023) / 1024
     // cost-model ordering on the device (nq_ctx_order_by_cost)
     nqk_buf::Buf<float> cm_params, cm_x, cm_y, cm_ys;
     nqk_buf::Buf<uint32_t> cm_idx;
@@ -100,6 +101,7 @@ int nq_check_ctx(nq_ctx *c);
 // (nq_count_noalloc).
 struct nq_run {
     uint32_t m = 0, Tmax = 0, Tmin = 0;
+    uint32_t fan = 1;             // expansion lanes per record (nitems / off hold m * fan entries)
     uint32_t split_min = 0;       // shallowest item start depth of the per-record splits
     const uint4 *root = nullptr, *sched = nullptr;
     const uint32_t *order = nullptr;   // processing order, or NULL for record order

@@ -288,8 +288,9 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 // walk_step scatters eleven such ops over the step.  This form keeps three
 // of them, in one dependent cluster (perm -> bfe -> lshl), and writes every
 // condition as an add/sub carry-out into an SGPR pair (x != 0: the carry of
-// x + 0xFFFFFFFF; b > a: the borrow of a - b), which the scalar unit then
-// combines.  The parameter byte comes from one perm over the two table words
+// x + 0xFFFFFFFF; b > a: the borrow of a - b; r != 0: the carry of the add
+// that forms the child's word, (r << 28) + 0xF0000000), which the scalar
+// unit then combines.  The parameter byte comes from one perm over the two table words
 // selected by W's sign (r >= 8), with selector byte 3 = r & 7.
 // The step is two asm blocks: the body, and the selects, which the compiler
 // places after its wait for the top entry; one block per step would need
@@ -340,12 +341,11 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
         "v_or_b32 %[b], %[nld], %[nrd]\n\t"
         "v_bitop3_b32 %[b], %[mask], %[b], %[ncol] bitop3:0x10\n\t"  // nf
         "v_and_b32 %[d], %[W], %[hi]\n\t"                        // r << 28
-        "v_add_u32 %[e], %[d], %[hi]\n\t"
+        "v_add_co_u32_e64 %[e], %[m3], %[d], %[hi]\n\t"         // (r - 1) << 28; carries when r != 0 (inner)
         "v_add_u32 %[Wc], %[e], %[b]\n\t"                        // the child's word
         // conditions
         "v_add_co_u32_e64 %[e], %[m1], %[b], -1\n\t"             // nf != 0
         "v_add_co_u32_e64 %[e], %[m2], %[a], -1\n\t"             // lookahead
-        "v_add_co_u32_e64 %[e], %[m3], %[d], -1\n\t"             // inner: r != 0
         "s_and_b64 %[m1], %[m1], %[m2]\n\t"                      // ok
         "v_sub_co_u32_e64 %[e], %[m2], %[d], %[rest]\n\t"        // more: rest > r << 28
         "s_and_b64 %[down], %[m1], %[m3]\n\t"

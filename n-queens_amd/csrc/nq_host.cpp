@@ -56,6 +56,7 @@ nq_run ctx_run(nq_ctx *c)
 {
     nq_run r;
     r.m = (uint32_t)c->m;
+    r.fan = c->fan;
     r.Tmax = c->Tmax;
     r.Tmin = c->Tmin;
     r.split_min = c->split_min;
@@ -152,9 +153,9 @@ int nq_batched_grid(int cu, const nq_sched &s, size_t lds_bytes, int nodes, uint
 int nq_count_items(hipStream_t st, const nq_run &r, int split, int block, uint64_t *count)
 {
     NQ_HIPCHK(nqk::launch_expand(st, false, false, block, expand_slots(r), r.root, r.sched, r.order,
-                                 split < 0 ? r.splits : nullptr, r.m, (uint32_t)(split < 0 ? 0 : split), r.nitems,
-                                 nullptr, nullptr, nullptr, nullptr));
-    NQ_HIPCHK(nqk::launch_scan(st, r.nitems, r.m, r.off, r.item_count, r.scan_tmp));
+                                 split < 0 ? r.splits : nullptr, r.m, (uint32_t)(split < 0 ? 0 : split), r.fan,
+                                 r.nitems, nullptr, nullptr, nullptr, nullptr));
+    NQ_HIPCHK(nqk::launch_scan(st, r.nitems, r.m * r.fan, r.off, r.item_count, r.scan_tmp, r.order, r.fan));
     unsigned long long cnt = 0;
     NQ_HIPCHK(hipMemcpyAsync(&cnt, r.item_count, sizeof cnt, hipMemcpyDeviceToHost, st));
     NQ_HIPCHK(hipStreamSynchronize(st));
@@ -162,8 +163,10 @@ int nq_count_items(hipStream_t st, const nq_run &r, int split, int block, uint64
     return 0;
 }
 
-// Every run recomputes the whole expansion from the resident records (the
-// sizing pass has only sized the item buffer).
+// Every run writes the whole expansion from the resident records.  The item
+// counts per (record, root candidate) are those of the sizing pass
+// (nq_count_items), kept by record with the resident set for its split depth;
+// the run scans them in its own processing order.
 int nq_enqueue_batched(hipStream_t st, const nq_run &r, const nq_sched &s, int nodes, int grid, hipEvent_t *ev)
 {
     const uint32_t m = r.m, sd = expand_slots(r);
@@ -172,14 +175,12 @@ int nq_enqueue_batched(hipStream_t st, const nq_run &r, const nq_sched &s, int n
     NQ_HIPCHK(nqk::launch_fill_u64(st, r.raw, m, 0));
     NQ_HIPCHK(nqk::launch_fill_u64(st, r.total, 1, 0));
     NQ_HIPCHK(nqk::launch_fill_u32(st, r.head, 1, 0));
-    if (!nodes) NQ_HIPCHK(nqk::launch_fill_u64(st, r.nodes, m, 0));
+    NQ_HIPCHK(nqk::launch_fill_u64(st, r.nodes, m, 0));
     const uint8_t *sp8 = s.split_depth < 0 ? r.splits : nullptr;
     const uint32_t su = (uint32_t)(s.split_depth < 0 ? 0 : s.split_depth);
-    NQ_HIPCHK(nqk::launch_expand(st, false, false, s.block, sd, r.root, r.sched, r.order, sp8, m,
-                                 su, r.nitems, nullptr, nullptr, nullptr, nullptr));
-    NQ_HIPCHK(nqk::launch_scan(st, r.nitems, m, r.off, r.item_count, r.scan_tmp));
+    NQ_HIPCHK(nqk::launch_scan(st, r.nitems, m * r.fan, r.off, r.item_count, r.scan_tmp, r.order, r.fan));
     NQ_HIPCHK(nqk::launch_expand(st, nodes != 0, true, s.block, sd, r.root, r.sched, r.order, sp8, m,
-                                 su, r.nitems, r.off, r.items, r.raw, r.nodes));
+                                 su, r.fan, r.nitems, r.off, r.items, r.raw, r.nodes));
     if (ev) NQ_HIPCHK(hipEventRecord(ev[1], st));
     NQ_HIPCHK(nqk::launch_batched(st, nodes != 0, s.lds_stack != 0, grid, s.block, lay, r.items,
                                   r.item_count, r.head, (uint32_t)s.chunk, r.raw, r.nodes));
@@ -335,18 +336,28 @@ int load_impl(nq_ctx *c, int N, const int32_t *Ns, const uint32_t *rec4, uint64_
             seen[order[a]] = 1;
         }
     }
+    // expansion lanes per record: one per root candidate, at most N of them
+    uint32_t fan = (uint32_t)N;
+    if (Ns) {
+        fan = 1;
+        for (uint8_t n : n8) fan = std::max<uint32_t>(fan, n);
+    }
+    if ((uint64_t)m * fan >= (1ull << 32))
+        return nq_fail(NQ_ECAPACITY, "%llu records of up to N=%u: the expansion takes fewer than 2^32 / N (load in chunks)",
+                       (unsigned long long)m, fan);
     drop_graph(c);
     c->loaded = false;
     c->sized_split = -2;
     c->has_splits = false;
+    c->fan = fan;
     const size_t mm = m ? m : 1;
     NQ_HIPCHK(c->rec.reserve(mm));
     NQ_HIPCHK(c->root.reserve(mm));
     NQ_HIPCHK(c->sched.reserve(nqk::kSchedU4 * mm));
     NQ_HIPCHK(c->order.reserve(mm));
-    NQ_HIPCHK(c->nitems.reserve(mm));
-    NQ_HIPCHK(c->off.reserve(mm));
-    NQ_HIPCHK(c->scan_tmp.reserve(mm / 1024 + 1));
+    NQ_HIPCHK(c->nitems.reserve(mm * fan));
+    NQ_HIPCHK(c->off.reserve(mm * fan));
+    NQ_HIPCHK(c->scan_tmp.reserve(mm * fan / 1024 + 1));
     NQ_HIPCHK(c->weighted.reserve(mm));
     NQ_HIPCHK(c->nodes.reserve(mm));
     NQ_HIPCHK(c->raw.reserve(mm));

@@ -37,15 +37,19 @@ hipError_t launch_decode(hipStream_t s, int N, const uint8_t *Ns, const uint4 *r
 hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const uint4 *root,
                         const uint4 *sched, uint32_t m, unsigned long long *weighted,
                         unsigned long long *nodes_out, unsigned long long *total);
+// one lane per (record position, root candidate < fan): nitems (by record)
+// and off (by position) hold m * fan entries (fan >= the set's largest N,
+// m * fan < 2^32)
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
                          const uint4 *root, const uint4 *sched, const uint32_t *order, const uint8_t *splits,
-                         uint32_t m, uint32_t split,
+                         uint32_t m, uint32_t split, uint32_t fan,
                          uint32_t *nitems, const uint32_t *off, uint4 *items,
                          unsigned long long *raw, unsigned long long *nodes_out);
 // exclusive scan of nitems[0..m) into off (u32) and the total into *count;
-// tsum: scratch of (m + 1023) / 1024 u64 entries
+// tsum: scratch of (m + 1023) / 1024 u64 entries.  With an order (records
+// of fan entries each), entry a is nitems[order[a / fan] * fan + a % fan].
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count,
-                       unsigned long long *tsum);
+                       unsigned long long *tsum, const uint32_t *order = nullptr, uint32_t fan = 0);
 hipError_t launch_preset(hipStream_t s, bool write, int N, const uint4 *in, uint32_t m, uint32_t *cnt,
                          const uint32_t *off, uint4 *out, uint32_t *err);
 hipError_t launch_batched(hipStream_t s, bool nodes, bool lds_stack, int grid, int block, WalkLds lay,

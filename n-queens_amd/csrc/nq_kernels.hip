@@ -107,19 +107,24 @@ k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_
 // ---------------------------------------------------------------------------
 // expansion: frames at depth `split` (one depth for all records, or a depth
 // per record) become work items
-//   pass 0 counts items per position q (processing order), pass 1 writes them
-//   at off[q].. ; nodes above the split depth are counted here.
+//   one lane per (processing position q, root candidate k < fan): the lane
+//   walks the root frame restricted to its record's k-th candidate, so a
+//   record's items come out in DFS order, k by k.  Pass 0 counts items per
+//   lane into nitems[c * fan + k] (by record, so the counts hold for any
+//   processing order), pass 1 writes them at off[q * fan + k]..; nodes above
+//   the split depth are counted here.
 // ---------------------------------------------------------------------------
 template <bool NODES, bool WRITE>
 __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict__ sched,
                          const uint32_t *__restrict__ order, const uint8_t *__restrict__ splits, uint32_t m,
-                         uint32_t split_all,
+                         uint32_t split_all, uint32_t fan,
                          uint32_t *__restrict__ nitems, const uint32_t *__restrict__ off,
                          uint4 *__restrict__ items,
                          unsigned long long *__restrict__ raw, unsigned long long *__restrict__ nodes_out)
 {
     extern __shared__ uint4 lds[];
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q = g / fan, k = g - q * fan;
     if (q >= m) return;
     const uint32_t c = order ? order[q] : q;
     const Sched S = load_sched(sched, c);
@@ -130,7 +135,7 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     SlotStack st{lds + threadIdx.x, (uint32_t)__builtin_ctz(blockDim.x)};
     uint64_t cnt = 0, nn = 0;
     uint32_t n = 0;
-    uint4 *const out = WRITE ? items + (size_t)kItemU4 * off[q] : nullptr;
+    uint4 *const out = WRITE ? items + (size_t)kItemU4 * off[g] : nullptr;
     auto put = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t w) {
         if (WRITE) {
             uint4 *it = out + (size_t)kItemU4 * n;
@@ -143,16 +148,22 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     if (r.w != 0u) {
         if (S.T < 2u || split == 0u || split > S.T - 2u) {
             // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
-            if (S.T) put(r.x, r.y, r.z, start_word(S, r.w, 0u));
-            else put(r.x, r.y, r.z, r.w);
+            if (k == 0u) {
+                if (S.T) put(r.x, r.y, r.z, start_word(S, r.w, 0u));
+                else put(r.x, r.y, r.z, r.w);
+            }
         } else {
-            const uint32_t upw = start_word(S, 0u, split);
-            auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | upw); };
-            expand_subtree<NODES>(S, r.x, r.y, r.z, r.w, 0u, split, st, cnt, nn, emit);
+            uint32_t av = r.w;
+            for (uint32_t i = 0; i < k && av; i++) av &= av - 1u;
+            if (av) {
+                const uint32_t upw = start_word(S, 0u, split);
+                auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | upw); };
+                expand_subtree<NODES>(S, r.x, r.y, r.z, av & (0u - av), 0u, split, st, cnt, nn, emit);
+            }
         }
     }
-    if (!WRITE) nitems[q] = n;
-    if (WRITE && NODES) nodes_out[c] = nn;   // expand nodes; DFS nodes are added atomically
+    if (!WRITE) nitems[c * fan + k] = n;
+    if (WRITE && NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);   // DFS nodes are added too
     (void)raw;
 }
 
@@ -181,7 +192,8 @@ __global__ void k_preset(int N, const uint4 *__restrict__ in, uint32_t m, uint32
 // exclusive scan of nitems[0..m) -> off (mod 2^32, as the item indices), total
 // -> *count, in three launches: each 1024-entry tile scanned by one block
 // (coalesced; u64 sums), the tile sums scanned by one block, the tile offsets
-// added back
+// added back.  With an order, entry a is read from nitems[order[a / fan] * fan
+// + a % fan] (record-indexed counts scanned in processing order).
 constexpr uint32_t kScanTile = 1024;
 
 __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x, uint32_t lane)
@@ -195,12 +207,21 @@ __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x, uint32_t lane
 }
 
 __global__ void __launch_bounds__(1024) k_scan_tile(const uint32_t *__restrict__ nitems, uint32_t m,
+                                                    const uint32_t *__restrict__ order, uint32_t fan,
                                                     uint32_t *__restrict__ off, unsigned long long *__restrict__ tsum)
 {
     __shared__ uint64_t wsum[kScanTile / 64];
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     const uint64_t a = (uint64_t)blockIdx.x * kScanTile + t;
-    const uint64_t v = a < m ? nitems[a] : 0u;
+    uint64_t v = 0;
+    if (a < m) {
+        if (order) {
+            const uint32_t q = (uint32_t)a / fan;
+            v = nitems[(uint64_t)order[q] * fan + ((uint32_t)a - q * fan)];
+        } else {
+            v = nitems[a];
+        }
+    }
     const uint64_t x = wave_incl_scan_u64(v, lane);
     if (lane == 63u) wsum[w] = x;
     __syncthreads();
@@ -550,15 +571,17 @@ hipError_t launch_plain(hipStream_t s, bool nodes, int block, WalkLds lay, const
 
 hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint32_t stack_depth,
                          const uint4 *root, const uint4 *sched, const uint32_t *order, const uint8_t *splits,
-                         uint32_t m, uint32_t split,
+                         uint32_t m, uint32_t split, uint32_t fan,
                          uint32_t *nitems, const uint32_t *off, uint4 *items,
                          unsigned long long *raw, unsigned long long *nodes_out)
 {
     if (m == 0) return hipSuccess;
+    if (fan == 0 || (uint64_t)m * fan >= (1ull << 32)) return hipErrorInvalidValue;
+    const uint32_t lanes = m * fan;
     const size_t lds = (size_t)block * stack_depth * sizeof(uint4);
-    const dim3 grid((m + block - 1) / block);
+    const dim3 grid((lanes + block - 1) / block);
 #define NQ_EXPAND(NB, WB) \
-    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, splits, m, split, nitems, off, items, raw, \
+    k_expand<NB, WB><<<grid, block, lds, s>>>(root, sched, order, splits, m, split, fan, nitems, off, items, raw, \
                                               nodes_out)
     if (write) { if (nodes) NQ_EXPAND(true, true); else NQ_EXPAND(false, true); }
     else { if (nodes) NQ_EXPAND(true, false); else NQ_EXPAND(false, false); }
@@ -567,11 +590,12 @@ hipError_t launch_expand(hipStream_t s, bool nodes, bool write, int block, uint3
 }
 
 hipError_t launch_scan(hipStream_t s, const uint32_t *nitems, uint32_t m, uint32_t *off, unsigned long long *count,
-                       unsigned long long *tsum)
+                       unsigned long long *tsum, const uint32_t *order, uint32_t fan)
 {
     const uint32_t tiles = (uint32_t)(((uint64_t)m + kScanTile - 1u) / kScanTile);
+    if (order && fan == 0) return hipErrorInvalidValue;
     if (tiles) {
-        k_scan_tile<<<tiles, kScanTile, 0, s>>>(nitems, m, off, tsum);
+        k_scan_tile<<<tiles, kScanTile, 0, s>>>(nitems, m, order, fan, off, tsum);
         if (hipError_t e = hipGetLastError()) return e;
     }
     k_scan_tops<<<1, 1024, 0, s>>>(tsum, tiles, count);
