@@ -31,7 +31,7 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgc
 // on the inert frame and waits at most this many steps (for its next item,
 // or for the wave's other lanes); the steps in between carry no wave-wide
 // check.
-constexpr int kStepsPerCheck = 32;
+constexpr int kStepsPerCheck = 64;
 
 // Count + nodes of a whole constellation from its root frame (depth 0).  A
 // lane whose walk has ended steps on the inert frame until every walk of its
