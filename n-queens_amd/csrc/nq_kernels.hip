@@ -146,12 +146,12 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
         n++;
     };
     if (r.w != 0u) {
-        if (S.T < 2u || split == 0u || split > S.T - 2u) {
-            // the root itself is the single item (T<2 roots are resolved by the DFS kernel)
-            if (k == 0u) {
-                if (S.T) put(r.x, r.y, r.z, start_word(S, r.w, 0u));
-                else put(r.x, r.y, r.z, r.w);
-            }
+        if (S.T == 0u) {
+            // a root on the terminal row: its one solution needs no walk
+            if (WRITE && k == 0u) raw[c] = (r.w & S.tm) ? 1u : 0u;
+        } else if (S.T < 2u || split == 0u || split > S.T - 2u) {
+            // the root itself is the single item (T = 1 roots are resolved by the DFS kernel)
+            if (k == 0u) put(r.x, r.y, r.z, start_word(S, r.w, 0u));
         } else {
             uint32_t av = r.w;
             for (uint32_t i = 0; i < k && av; i++) av &= av - 1u;
@@ -164,7 +164,6 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
     }
     if (!WRITE) nitems[c * fan + k] = n;
     if (WRITE && NODES && nn) atomicAdd(nodes_out + c, (unsigned long long)nn);   // DFS nodes are added too
-    (void)raw;
 }
 
 // ---------------------------------------------------------------------------
@@ -314,23 +313,22 @@ k_batched(const uint4 *__restrict__ items, const unsigned long long *__restrict_
         const bool inert = w.W == 0u;   // a walking frame always has candidates
         if (wave_ballot(inert || cnt >= kCountFlush) == 0ull) continue;
         if (have && inert) {
-            // the walk ended: hand in its counts after the switch below
-            fc = c; fcnt = cnt; fnn = nn; pend = true;
-            cnt = 0; have = false;
+            // the walk ended; its counts go on with the held item when that
+            // is of the same record, else they are handed in after the switch
+            have = false;
+            if (!held || p1.x != c) {
+                fc = c; fcnt = cnt; fnn = nn; pend = true;
+                cnt = 0; nn = 0;
+            }
         }
-        // switch to the held item (its loads were issued at an earlier check)
+        // switch to the held item (its loads were issued at an earlier check);
+        // every item starts a walk (roots on the terminal row make no item)
         if (!have && held) {
             held = false;
             c = p1.x;
-            const uint32_t T = p1.y & 0xFFu, N = (p1.y >> 16) & 31u;
-            cnt = 0; nn = 0;
-            if (T == 0u) {
-                if (p0.w & p1.z) atomicAdd(raw + c, 1ull);   // a root on the terminal row
-            } else if ((p0.w & kAvailMask) != 0u) {
-                L = lane_k(N, p2.x, p2.y, p2.z, p2.w);
-                w = Walk{p0.x, p0.y, p0.z, p0.w, st.empty()};
-                have = true;
-            }
+            L = lane_k((p1.y >> 16) & 31u, p2.x, p2.y, p2.z, p2.w);
+            w = Walk{p0.x, p0.y, p0.z, p0.w, st.empty()};
+            have = true;
         }
         // hand out the next items of the wave's range to the lanes without a held item
         while (!exhausted) {
