@@ -109,7 +109,8 @@ k_plain(const uint4 *__restrict__ root, const uint4 *__restrict__ sched, uint32_
 // per record) become work items
 //   one lane per (processing position q, root candidate k < fan): the lane
 //   walks the root frame restricted to its record's k-th candidate, so a
-//   record's items come out in DFS order, k by k.  Pass 0 counts items per
+//   record's items come out in DFS order, k by k (fan 1: one lane walks all
+//   of a record's candidates, with per-record arrays).  Pass 0 counts items per
 //   lane into nitems[c * fan + k] (by record, so the counts hold for any
 //   processing order), pass 1 writes them at off[q * fan + k]..; nodes above
 //   the split depth are counted here.
@@ -158,7 +159,9 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
             if (av) {
                 const uint32_t upw = start_word(S, 0u, split);
                 auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | upw); };
-                expand_subtree<NODES>(S, r.x, r.y, r.z, av & (0u - av), 0u, split, st, cnt, nn, emit);
+                // fan 1: the lane walks every candidate of its root
+                expand_subtree<NODES>(S, r.x, r.y, r.z, fan == 1u ? av : av & (0u - av), 0u, split, st, cnt, nn,
+                                      emit);
             }
         }
     }

@@ -104,7 +104,7 @@ extern "C" int nq_count_noalloc(nq_ctx *c, int N, const uint32_t *rec4, uint64_t
     uint4 *root = cv.take<uint4>(m), *sch = cv.take<uint4>(nqk::kSchedU4 * m);
     r.root = root;
     r.sched = sch;
-    r.fan = (uint32_t)N;
+    r.fan = 1u;   // one expansion lane per record: per-record count arrays leave the array's room to work items
     r.nitems = cv.take<uint32_t>(m * r.fan);
     r.off = cv.take<uint32_t>(m * r.fan);
     r.scan_tmp = cv.take<unsigned long long>(m * r.fan / 1024 + 1);
