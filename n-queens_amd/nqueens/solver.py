@@ -152,8 +152,12 @@ def count_chunks(N: int, chunks: int, first: int = 0, last: int | None = None, *
         raise ValueError(f"unknown scheduler {scheduler!r}")
     ctx = ctx or default_context()
     preset = preset or preset_for(N)
-    base = _cons.gen_constellations(N, min(preset, 5))
-    levels = preset - min(preset, 5)
+    # One chunk is the whole set: the host generates preset 4 (a tenth of
+    # preset 5's records at N=19, a fifth of its time) and the device places
+    # the rest; the same multiset of records, in another order.
+    host_preset = min(preset, 4 if ids == [0] and chunks == 1 else 5)
+    base = _cons.gen_constellations(N, host_preset)
+    levels = preset - host_preset
     if mode == "batched" and sched is None:
         from .autotune import best_sched
         sched = best_sched(N, preset)

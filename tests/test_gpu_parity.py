@@ -259,7 +259,8 @@ def test_count_in_chunks(ctx, known):
     assert sum(count_chunks(16, 4, preset=7, ctx=ctx)) == total
 
 
-@pytest.mark.parametrize("N,p,levels", [(N, 5, 1) for N in (8, 12, 16, 19, 21, 23)] + [(14, 4, 1), (13, 5, 2), (16, 5, 3)])
+@pytest.mark.parametrize("N,p,levels", [(N, 5, 1) for N in (8, 12, 16, 19, 21, 23)] + [(N, 4, 1) for N in (6, 7, 14, 19, 22)]
+                         + [(13, 5, 2), (16, 5, 3)])
 def test_device_preset_expansion(ctx, N, p, levels):
     """nq_expand_records gives exactly the host generator's records of the
     deeper preset (as multisets; the device keeps record order, columns

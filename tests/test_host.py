@@ -27,6 +27,17 @@ def test_generator_matches_oracle(oracle, N, preset):
     assert records_sha256(mine) == records_sha256(theirs)
 
 
+@pytest.mark.parametrize("N", range(6, 20))
+def test_preset4_plus_one_level_is_preset5(oracle, digests, N):
+    """A one-chunk count generates preset 4 on the host and places the fifth
+    queen on the device: one level of preset-4 records (the oracle's
+    expansion, which the device's is tested against) is the preset-5 set of
+    the reference's fixture, as a multiset."""
+    kids = oracle.expand_records(N, C.gen_constellations(N, 4), 1)
+    assert kids.shape[0] == digests[str(N)]["constellations"]
+    assert np.array_equal(C.records_sorted(kids), C.records_sorted(C.gen_constellations(N, 5)))
+
+
 @pytest.mark.parametrize("N", [8, 11, 12, 17, 20, 21])
 def test_start_order_matches_oracle(oracle, N):
     """The start constellations come in the reference's insertion order (the
