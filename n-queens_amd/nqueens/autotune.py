@@ -19,7 +19,9 @@ import statistics
 import time
 from pathlib import Path
 
-from ._lib import Context, Sched
+from ._lib import Context, NQError, Sched
+
+NQ_ECAPACITY = -3   # nq_status of a schedule beyond the library's limits
 
 __all__ = ["TUNED", "TUNED_PRESETS", "best_sched", "tuned_preset", "sweep", "sweep_presets", "sched_from"]
 
@@ -77,23 +79,32 @@ def sweep(ctx: Context, N: int, expect_total: int, splits=(2, 3, 4, 5, 6), block
     """Time every configuration on the task set loaded in `ctx`.
 
     Returns one row per configuration: its parameters, the work-item count and
-    the median/best DFS-kernel and whole-run device times (ms)."""
+    the median/best DFS-kernel and whole-run device times (ms).  A
+    configuration the library refuses for capacity (a split whose items
+    overflow the u32 item indices, a stack larger than the LDS) gives a row
+    with its error and no times."""
     rows = []
     for split, block, st, chunk in itertools.product(splits, blocks, lds, chunks):
         s = Sched(split_depth=split, block=block, lds_stack=st, use_graph=0, chunk=chunk, waves_per_eu=0)
-        for _ in range(warmup):
-            tot, _ = ctx.run_batched(s)
-            if tot != expect_total:
-                raise RuntimeError(f"autotune: wrong total {tot} at {split},{block},{st}")
         ks, rs = [], []
         items = 0
-        for _ in range(reps):
-            tot, t = ctx.run_batched(s)
-            if tot != expect_total:
-                raise RuntimeError(f"autotune: wrong total {tot} at {split},{block},{st}")
-            ks.append(t.kernel_ms)
-            rs.append(t.total_ms)
-            items = int(t.items)
+        try:
+            for i in range(warmup + reps):
+                tot, t = ctx.run_batched(s)
+                if tot != expect_total:
+                    raise RuntimeError(f"autotune: wrong total {tot} at {split},{block},{st}")
+                if i >= warmup:
+                    ks.append(t.kernel_ms)
+                    rs.append(t.total_ms)
+                    items = int(t.items)
+        except NQError as e:
+            if e.code != NQ_ECAPACITY:
+                raise
+            row = {"split_depth": split, "block": block, "lds_stack": st, "chunk": chunk, "error": str(e)}
+            rows.append(row)
+            if log:
+                log(row)
+            continue
         row = {"split_depth": split, "block": block, "lds_stack": st, "chunk": chunk, "items": items,
                "kernel_ms_median": statistics.median(ks), "kernel_ms_best": min(ks),
                "run_ms_median": statistics.median(rs), "run_ms_best": min(rs)}
@@ -169,8 +180,12 @@ def record_presets(N: int, rows: list[dict], path: Path = PRESETS_PATH, tuned_pa
 
 
 def record_best(N: int, preset: int, rows: list[dict], path: Path = TUNED_PATH) -> dict:
-    """Store the configuration with the lowest median whole-run time."""
-    best = min(rows, key=lambda r: r["run_ms_median"])
+    """Store the configuration with the lowest median whole-run time (rows
+    with an error are skipped)."""
+    ok = [r for r in rows if "error" not in r]
+    if not ok:
+        raise ValueError(f"N={N} preset {preset}: no configuration could be counted")
+    best = min(ok, key=lambda r: r["run_ms_median"])
     table = _load(path)
     table[f"{N},{preset}"] = {k: best[k] for k in ("split_depth", "block", "lds_stack", "chunk")} | {
         "run_ms_median": round(best["run_ms_median"], 3)}

@@ -122,3 +122,33 @@ def test_sweep_presets_control_flow(monkeypatch):
     assert ctx.ordered == 2 and all(lds == 1 for _, _, lds in ctx.runs)
     with pytest.raises(RuntimeError):    # a wrong total is never recorded
         autotune.sweep_presets(ctx, 13, 1, presets=(4,), splits=(2,), blocks=(64,), reps=1, warmup=0)
+
+
+def test_sweep_skips_capacity_refusals(tmp_path, monkeypatch):
+    """A split whose items overflow the library's u32 indices is refused with
+    NQ_ECAPACITY: the sweep reports that point and goes on; any other error,
+    and a wrong total, still stop it; the refused point is never stored."""
+    from nqueens import _lib
+
+    class FakeCtx:
+        m = 1
+
+        def __init__(self, code=-3):
+            self.code = code
+
+        def run_batched(self, s):
+            if s.split_depth == 5:
+                raise _lib.NQError(self.code, "nq_ctx_run_batched", "split depth 5 gives 9348244174 work items")
+            return 314666222712, _lib.Timing(kernel_ms=float(s.split_depth), total_ms=float(s.split_depth) + 1,
+                                             items=10 ** s.split_depth, launches=1)
+
+    monkeypatch.setattr(autotune, "TUNED", {})   # record_best refreshes the module table
+    rows = autotune.sweep(FakeCtx(), 21, 314666222712, splits=(3, 4, 5), blocks=(256,), lds=(1,), reps=1)
+    assert [r["split_depth"] for r in rows] == [3, 4, 5]
+    assert "error" in rows[2] and "NQ_ECAPACITY" in rows[2]["error"] and "kernel_ms_median" not in rows[2]
+    best = autotune.record_best(21, 5, rows, tmp_path / "tuned.json")
+    assert best["split_depth"] == 3
+    with pytest.raises(_lib.NQError):
+        autotune.sweep(FakeCtx(code=-4), 21, 314666222712, splits=(5,), blocks=(256,), lds=(1,), reps=1)
+    with pytest.raises(ValueError):
+        autotune.record_best(21, 5, rows[2:], tmp_path / "tuned.json")
