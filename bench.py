@@ -193,9 +193,10 @@ def main() -> int:
     achieved = nodes_total * OPS_PER_NODE / (kavg / 1000.0) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(PEAK_INT32_TOPS, 2),
                 "unit": "TOP/s", "frac": round(achieved / PEAK_INT32_TOPS, 4),
-                "traffic": profiled_traffic(N, preset, sched),
+                "traffic": profiled_figure(N, preset, "traffic_bytes_per_launch", int),
                 "kernel": "k_batched", "kernel_ms": round(kavg, 3),
-                "ops_per_node": OPS_PER_NODE, "nodes_per_launch": nodes_total}
+                "ops_per_node": OPS_PER_NODE, "nodes_per_launch": nodes_total,
+                "issue_probe": issue_probe(ctx, N, preset, nodes_total, kavg)}
 
     big = None
     if args.big_n:
@@ -234,16 +235,37 @@ def main() -> int:
     return 0
 
 
-def profiled_traffic(N: int, preset: int, sched):
-    """HBM bytes per k_batched launch from the committed rocprofv3 FETCH_SIZE +
-    WRITE_SIZE summary of this workload (None when no summary matches)."""
+def profiled_figure(N: int, preset: int, key: str, conv=float):
+    """A figure of the committed rocprofv3 summary of this workload
+    (`key: value` line of profiles/k_batched_N{N}_p{preset}.md): the HBM bytes
+    per k_batched launch (FETCH_SIZE + WRITE_SIZE), the VALU instructions per
+    node (SQ_INSTS_VALU per wave step); None when no summary matches."""
     p = ROOT / "profiles" / f"k_batched_N{N}_p{preset}.md"
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
+        if line.startswith(key + ":"):
+            return conv(line.split(":")[1])
     return None
+
+
+def issue_probe(ctx, N: int, preset: int, nodes: int, kernel_ms: float) -> dict:
+    """The VALU issue ceiling measured live (nq_issue_probe: independent
+    v_or_b32 / v_and_b32 chains at the DFS kernel's four waves per SIMD), and
+    the DFS kernel's own VALU issue rate against it: the profiled VALU
+    instructions per node times nodes per second of the event-timed kernel."""
+    ctx.issue_probe(4, 2000)
+    p = ctx.issue_probe(4, 40000)
+    out = {"ops": "v_or_b32 / v_and_b32, 8 independent chains per lane", "waves_per_simd": 4,
+           "clk_per_wave_instr_per_simd": round(p["clk_per_wave_instr_per_simd"], 3),
+           "lane_ops_per_s": round(p["lane_ops_per_s"], 1),
+           "frac_of_peak": round(p["lane_ops_per_s"] / (PEAK_INT32_TOPS * 1e12), 4)}
+    vpn = profiled_figure(N, preset, "valu_per_node")
+    if vpn:
+        kernel_ops = vpn * nodes / (kernel_ms / 1000.0)
+        out.update({"kernel_valu_per_node": vpn, "kernel_lane_ops_per_s": round(kernel_ops, 1),
+                    "kernel_frac_of_probe": round(kernel_ops / p["lane_ops_per_s"], 4)})
+    return out
 
 
 def big_count(nqueens, ctx, N: int, preset: int, scheduler: str, plain: bool) -> dict:

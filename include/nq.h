@@ -72,6 +72,14 @@ int nq_ctx_destroy(nq_ctx *ctx);
 /* device properties the scheduler uses: {compute units, max LDS bytes per
  * workgroup, wavefront size, clock kHz} */
 int nq_device_info(nq_ctx *ctx, int32_t *out4);
+/* No counterpart in the reference (a measurement for the roofline): the VALU
+ * issue rate of independent v_or_b32 / v_and_b32 chains with waves_per_simd
+ * (1..8) waves resident on every SIMD, `iters` (1..2^24) iterations of 32
+ * ops per lane.  *ms_out: device time; *clk_out: clocks per wave instruction
+ * per SIMD at the reported clock; *xor_out: XOR of every lane's result (a
+ * check that the chains ran).  Any out pointer may be NULL. */
+int nq_issue_probe(nq_ctx *ctx, int waves_per_simd, uint32_t iters, double *ms_out, double *clk_out,
+                   uint32_t *xor_out);
 
 /* ---- resident task set ---------------------------------------------------
  * exec_solutions' input list (18Py:1057): copies m records to HBM and decodes

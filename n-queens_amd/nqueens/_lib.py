@@ -59,6 +59,8 @@ SIGNATURES = {
     "nq_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "nq_ctx_destroy": (ctypes.c_int, [_vp]),
     "nq_device_info": (ctypes.c_int, [_vp, _i32p]),
+    "nq_issue_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32)]),
     "nq_ctx_load": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, _vp]),
     "nq_ctx_load_mixed": (ctypes.c_int, [_vp, _i32p, _u32p, ctypes.c_uint64, _vp]),
     "nq_expand_records": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.c_int, _vp,
@@ -163,6 +165,19 @@ class Context:
         check(self.lib.nq_device_info(self.h, out), "nq_device_info")
         return {"compute_units": int(out[0]), "lds_bytes": int(out[1]), "wavefront": int(out[2]),
                 "clock_khz": int(out[3])}
+
+    def issue_probe(self, waves_per_simd: int = 4, iters: int = 40000) -> dict:
+        """VALU issue rate of independent v_or/v_and chains (nq_issue_probe):
+        device ms, clocks per wave instruction per SIMD, lane-ops per second
+        over the whole device, and the XOR of the lanes' results."""
+        ms, clk, x = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_uint32(0)
+        check(self.lib.nq_issue_probe(self.h, int(waves_per_simd), int(iters), ctypes.byref(ms),
+                                      ctypes.byref(clk), ctypes.byref(x)), "nq_issue_probe")
+        d = self.device_info()
+        simds = 4 * d["compute_units"]
+        lane_ops = 64.0 * simds * d["clock_khz"] * 1e3 / clk.value if clk.value > 0 else 0.0
+        return {"waves_per_simd": int(waves_per_simd), "iters": int(iters), "ms": ms.value,
+                "clk_per_wave_instr_per_simd": clk.value, "lane_ops_per_s": lane_ops, "xor": int(x.value)}
 
     def load(self, N: int, rec, order=None) -> None:
         rec = as_records(rec)

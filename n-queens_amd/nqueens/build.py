@@ -15,7 +15,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG.parent / "csrc"
 ROOT = PKG.parent.parent
 LIB = PKG / "libnq.so"
-SOURCES = ["nq_kernels.hip", "nq_mlp.hip", "nq_noalloc.hip", "nq_host.cpp"]
+SOURCES = ["nq_kernels.hip", "nq_mlp.hip", "nq_noalloc.hip", "nq_probe.hip", "nq_host.cpp"]
 HEADERS = ["nq_task.h", "nq_dfs.h", "nq_kernels.h", "nq_ctx.h"]
 ARCH = os.environ.get("NQ_OFFLOAD_ARCH", "gfx950")
 
