@@ -290,8 +290,9 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 // condition as an add/sub carry-out into an SGPR pair (x != 0: the carry of
 // x + 0xFFFFFFFF; b > a: the borrow of a - b; r != 0: the carry of the add
 // that forms the child's word, (r << 28) + 0xF0000000), which the scalar
-// unit then combines.  The parameter byte comes from one perm over the two table words
-// selected by W's sign (r >= 8), with selector byte 3 = r & 7.
+// unit then combines.  The parameter byte comes from one perm: W + W drops
+// r's top bit and carries it out (r >= 8), the carry picks the pair of table
+// words by two selects, and (W + W) >> 5 is the selector, byte 3 = r & 7.
 // The step is two asm blocks: the body, and the selects, which the compiler
 // places after its wait for the top entry; one block per step would need
 // the frame in a register quadruple for the stack write.  Operands and
@@ -299,7 +300,6 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 struct PairK {
     uint32_t hi;      // 0xF0000000: the r field
     uint32_t S, nS;   // the stack's slot stride and its negative
-    uint32_t sel;     // 0x07FFFFFF: keeps r & 7 in byte 3 of the perm selector
 };
 #if defined(__HIP_DEVICE_COMPILE__)
 template <bool NODES, class Stack>
@@ -312,11 +312,10 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
     uint64_t m1, m2, m3, up, down, push, pop;
     asm volatile(
         // byte lookup and the ld shift: the one cluster of slow ops
-        "v_ashrrev_i32 %[c], 31, %[W]\n\t"                        // r >= 8
-        "v_lshrrev_b32 %[a], 4, %[W]\n\t"
-        "v_bitop3_b32 %[b], %[c], %[t2], %[t0] bitop3:0xca\n\t"  // table words of r's half
-        "v_bitop3_b32 %[e], %[c], %[t3], %[t1] bitop3:0xca\n\t"
-        "v_and_b32 %[a], %[a], %[sel]\n\t"                       // selector, byte 3 = r & 7
+        "v_add_co_u32_e64 %[a], %[m1], %[W], %[W]\n\t"           // W << 1; the carry: r >= 8
+        "v_cndmask_b32_e64 %[b], %[t0], %[t2], %[m1]\n\t"        // table words of r's half
+        "v_lshrrev_b32 %[a], 5, %[a]\n\t"                        // selector, byte 3 = r & 7
+        "v_cndmask_b32_e64 %[e], %[t1], %[t3], %[m1]\n\t"
         "v_sub_u32 %[d], 0, %[W]\n\t"
         "v_and_b32 %[d], %[W], %[d]\n\t"                         // bit
         "v_or_b32 %[f], %[ld], %[d]\n\t"
@@ -361,7 +360,7 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
           [pop] "=&s"(pop), [cnt] "+v"(cnt)
         : [W] "v"(W), [ld] "v"(ld), [rd] "v"(rd), [col] "v"(C), [A] "v"(A), [E] "v"(E), [t0] "v"(L.t0),
           [t1] "v"(L.t1), [t2] "v"(L.t2), [t3] "v"(L.t3), [ksh] "v"(L.ksh), [km] "v"(L.km), [three] "v"(L.three),
-          [mask] "v"(L.mask), [nlab] "v"(L.nlab), [hi] "v"(K.hi), [sel] "v"(K.sel)
+          [mask] "v"(L.mask), [nlab] "v"(L.nlab), [hi] "v"(K.hi)
         : "scc");
     if (NODES) nodes += (uint64_t)(W != 0u);             // the inert frame is no node
     uint32_t l1, r1, c1, w1, dA;
@@ -394,7 +393,7 @@ NQ_HD inline void walk_step_paired(const LaneK &L, const PairK &, Walk &w, Stack
 #endif
 inline NQ_HD PairK pair_consts(uint32_t S)
 {
-    return PairK{vconst(0xF0000000u), vconst(S), vconst(0u - S), vconst(0x07FFFFFFu)};
+    return PairK{vconst(0xF0000000u), vconst(S), vconst(0u - S)};
 }
 
 // ---------------------------------------------------------------------------
