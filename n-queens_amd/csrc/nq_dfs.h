@@ -52,8 +52,9 @@ NQ_HD inline Sched load_sched(const uint4 *sched, uint32_t c)
 // ---------------------------------------------------------------------------
 // frames
 // ---------------------------------------------------------------------------
-// A frame is {ld, rd, col, W}: the diagonals, the occupied columns (board
-// bits 0..N-1, N <= 28) and the frame word W, which holds the candidates
+// A frame is {ld, rd, fc, W}: the diagonals, the free columns (the board's
+// columns 0..N-1, N <= 28, that hold no queen: mask & ~col) and the frame
+// word W, which holds the candidates
 // (avail, bits 0-27) and r = T-1-depth (bits 28-31).  The frame's children
 // are on the terminal row exactly when r = 0, i.e. when W is below 2^28; a
 // child's word is W's top bits less 2^28 plus its candidates; and the
@@ -64,6 +65,9 @@ constexpr uint32_t kQShift = 24u;
 constexpr uint32_t kRShift = 28u;
 
 NQ_HD inline uint32_t frame_word(uint32_t avail, uint32_t r) { return avail | (r << kRShift); }
+
+// The frame's free columns from the occupied ones.
+NQ_HD inline uint32_t free_cols(const Sched &S, uint32_t col) { return S.mask & ~col; }
 
 // Frame word of a frame at depth d with candidates avail.
 NQ_HD inline uint32_t start_word(const Sched &S, uint32_t avail, uint32_t d)
@@ -104,25 +108,24 @@ NQ_HD inline uint32_t vconst(uint32_t v)
     return v;
 }
 
-// Per-lane constants of the walk: board mask, rd shift, the parameter bytes
-// by r and the masks below (in VGPRs).  An idle lane (mask 0) steps without
-// effect: no candidate, no count, no stack motion.
+// Per-lane constants of the walk: rd shift, the parameter bytes by r and the
+// masks below (in VGPRs).  A lane on the inert frame steps without effect:
+// no candidate, no count, no stack motion.
 // With the frame's parameter byte q in bits 24-31 of a word, the rd constant
 // of the child is (q >> (32-N)) & km, km = 0b1011 << (N-4): ARc (bits 28-29)
 // to N-4.., JD (bit 31) to N-1.
 constexpr uint32_t kNlaBit = 1u << 30;   // the NLA flag of q
 struct LaneK {
-    uint32_t mask;
     uint32_t ksh, km;     // 32 - N and 0b1011 << (N-4)
     uint32_t t0, t1, t2, t3;
     uint32_t nlab, three;
 };
 NQ_HD inline LaneK lane_k(uint32_t N, uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3)
 {
-    return LaneK{(1u << N) - 1u, 32u - N, 0xBu << (N - 4u), t0, t1, t2, t3, vconst(kNlaBit), vconst(3u)};
+    return LaneK{32u - N, 0xBu << (N - 4u), t0, t1, t2, t3, vconst(kNlaBit), vconst(3u)};
 }
 NQ_HD inline LaneK lane_consts(const Sched &S) { return lane_k(S.nsh + 4u, S.t[0], S.t[1], S.t[2], S.t[3]); }
-NQ_HD inline LaneK idle_lane() { return LaneK{0u, 28u, 3u, 0u, 0u, 0u, 0u, vconst(kNlaBit), vconst(3u)}; }
+NQ_HD inline LaneK idle_lane() { return LaneK{28u, 3u, 0u, 0u, 0u, 0u, vconst(kNlaBit), vconst(3u)}; }
 
 // Parameter byte of a frame, in bits 24-31 (bits 0-23 are not defined), from
 // its frame word (top bits: the frame's r).  On the device two v_perm_b32
@@ -166,13 +169,13 @@ NQ_HD inline uint32_t lds_addr(const void *) { return 0u; }
 #endif
 
 // ---------------------------------------------------------------------------
-// stacks of 16-byte frames {ld, rd, col, frame word}, byte-addressed:
+// stacks of 16-byte frames {ld, rd, fc, frame word}, byte-addressed:
 // slot i of a lane at base + i*S.  The slot below base holds the inert frame
-// {0, 0, all columns, no candidates}: the top entry of an empty stack.  The
+// {0, 0, no free column, no candidates}: the top entry of an empty stack.  The
 // walk keeps the address of its top entry (base - S when empty); a walk that
 // ends takes the inert frame and stays on it, stepping without effect.
 // ---------------------------------------------------------------------------
-NQ_HD inline uint4 inert_frame() { return make_uint4(0u, 0u, ~0u, 0u); }
+NQ_HD inline uint4 inert_frame() { return make_uint4(0u, 0u, 0u, 0u); }
 
 struct LdsStack {
     LdsMem m;
@@ -196,7 +199,7 @@ struct PrivStack {
 // A lane's DFS position: the current frame and the address of the top
 // stack entry.
 struct Walk {
-    uint32_t ld, rd, col, W;
+    uint32_t ld, rd, fc, W;
     uint32_t A;
 };
 
@@ -220,7 +223,7 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 {
     const uint32_t S = st.S;
     const uint4 top = st.read(w.A);
-    const uint32_t W = w.W, C = w.col;
+    const uint32_t W = w.W, F = w.fc;
     const uint32_t q = frame_q(L, W);                    // this frame's parameter byte (bits 24-31)
     const uint32_t bit = W & (0u - W);                   // lowest candidate (avail is the low field)
     const uint32_t rest = W ^ bit;
@@ -228,8 +231,8 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const uint32_t x = (w.ld | bit) << s;                // the child's ld before its constants
     const uint32_t nld = bop3<kOrAnd>(x, q >> 26, L.three);          // | AL
     const uint32_t nrd = bop3<kOrAnd>((w.rd | bit) >> s, q >> L.ksh, L.km);
-    const uint32_t ncol = C | bit;
-    uint32_t nf = bop3<kAndNotOr>(L.mask, nld | nrd, ncol);
+    const uint32_t nfc = F & ~bit;                       // the child's free columns
+    uint32_t nf = bop3<kAndNotOr>(nfc, nld, nrd);
     // nonzero when the frame has no prune
     const uint32_t nla = q & L.nlab;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -241,8 +244,8 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 #else
     const uint32_t x2 = x << 1;
 #endif
-    const uint32_t look = bop3<kOr3>(x2, nrd >> 1, ncol);
-    const bool lok = bop3<kAndNotOrC>(L.mask, look, nla) != 0u;
+    // the next row keeps a free square (or the frame has no prune)
+    const bool lok = (bop3<kAndNotOr>(nfc, x2, nrd >> 1) | nla) != 0u;
     const uint32_t hiW = W & ~kAvailMask;                // r << 28
     const bool inner = W >= (1u << kRShift);             // children not on the terminal row
     const bool ok = lok & (nf != 0u);
@@ -258,19 +261,19 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const bool up = !down & !more;                       // take the top entry
     const bool nonempty = w.A != st.empty();
     const bool pop = up & nonempty;                      // ... and drop it from the stack
-    const uint4 rem = make_uint4(w.ld, w.rd, C, rest);
+    const uint4 rem = make_uint4(w.ld, w.rd, F, rest);
     const uint32_t A = w.A;
     // the child: r - 1, candidates nf
     const uint32_t Wc = hiW + nf - (1u << kRShift);
     w.ld = down ? nld : (up ? top.x : w.ld);
     w.rd = down ? nrd : (up ? top.y : w.rd);
-    w.col = down ? ncol : (up ? top.z : C);
+    w.fc = down ? nfc : (up ? top.z : F);
     w.W = down ? Wc : (up ? top.w : rest);
     w.A += push ? S : (pop ? 0u - S : 0u);
     // written after the selects: the wait for the top entry does not also
     // wait for this write
 #if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("" ::"v"(w.ld), "v"(w.rd), "v"(w.col), "v"(w.W));
+    asm volatile("" ::"v"(w.ld), "v"(w.rd), "v"(w.fc), "v"(w.W));
 #endif
     if (push) st.write(A + S, rem);
     return up & !nonempty;
@@ -307,8 +310,8 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
                                         uint64_t &nodes)
 {
     const uint4 top = st.read(w.A);
-    const uint32_t W = w.W, ld = w.ld, rd = w.rd, C = w.col, A = w.A, E = st.empty();
-    uint32_t nld, nrd, ncol, rest, Wc, ta, tb, tc, td, te, tf;
+    const uint32_t W = w.W, ld = w.ld, rd = w.rd, F = w.fc, A = w.A, E = st.empty();
+    uint32_t nld, nrd, nfc, rest, Wc, ta, tb, tc, td, te, tf;
     uint64_t m1, m2, m3, up, down, push, pop;
     asm volatile(
         // byte lookup and the ld shift: the one cluster of slow ops
@@ -329,16 +332,15 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
         "v_bitop3_b32 %[nld], %[a], %[b], %[three] bitop3:0xf8\n\t"
         "v_lshrrev_b32 %[b], %[ksh], %[c]\n\t"
         "v_bitop3_b32 %[nrd], %[e], %[b], %[km] bitop3:0xf8\n\t"
-        "v_or_b32 %[ncol], %[col], %[d]\n\t"
+        "v_bitop3_b32 %[nfc], %[fc], %[d], %[d] bitop3:0x30\n\t"  // the child's free columns: fc & ~bit
         "v_xor_b32 %[rest], %[W], %[d]\n\t"
         // the one-row lookahead and the child's free set
         "v_add_u32 %[a], %[a], %[a]\n\t"
         "v_lshrrev_b32 %[b], 1, %[nrd]\n\t"
-        "v_bitop3_b32 %[a], %[a], %[b], %[ncol] bitop3:0xfe\n\t"
+        "v_bitop3_b32 %[a], %[nfc], %[a], %[b] bitop3:0x10\n\t"   // free squares of the next row
         "v_and_b32 %[c], %[c], %[nlab]\n\t"
-        "v_bitop3_b32 %[a], %[mask], %[a], %[c] bitop3:0xba\n\t"  // lookahead passes (or none here)
-        "v_or_b32 %[b], %[nld], %[nrd]\n\t"
-        "v_bitop3_b32 %[b], %[mask], %[b], %[ncol] bitop3:0x10\n\t"  // nf
+        "v_or_b32 %[a], %[a], %[c]\n\t"                          // lookahead passes (or none here)
+        "v_bitop3_b32 %[b], %[nfc], %[nld], %[nrd] bitop3:0x10\n\t"  // nf
         "v_and_b32 %[d], %[W], %[hi]\n\t"                        // r << 28
         "v_add_co_u32_e64 %[e], %[m3], %[d], %[hi]\n\t"         // (r - 1) << 28; carries when r != 0 (inner)
         "v_add_u32 %[Wc], %[e], %[b]\n\t"                        // the child's word
@@ -354,13 +356,13 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
         "s_nor_b64 %[up], %[down], %[m2]\n\t"
         "s_and_b64 %[pop], %[up], %[m3]\n\t"
         "v_addc_co_u32_e64 %[cnt], %[m2], 0, %[cnt], %[m1]"     // cnt += sol
-        : [nld] "=&v"(nld), [nrd] "=&v"(nrd), [ncol] "=&v"(ncol), [rest] "=&v"(rest), [Wc] "=&v"(Wc),
+        : [nld] "=&v"(nld), [nrd] "=&v"(nrd), [nfc] "=&v"(nfc), [rest] "=&v"(rest), [Wc] "=&v"(Wc),
           [a] "=&v"(ta), [b] "=&v"(tb), [c] "=&v"(tc), [d] "=&v"(td), [e] "=&v"(te), [f] "=&v"(tf),
           [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [up] "=&s"(up), [down] "=&s"(down), [push] "=&s"(push),
           [pop] "=&s"(pop), [cnt] "+v"(cnt)
-        : [W] "v"(W), [ld] "v"(ld), [rd] "v"(rd), [col] "v"(C), [A] "v"(A), [E] "v"(E), [t0] "v"(L.t0),
+        : [W] "v"(W), [ld] "v"(ld), [rd] "v"(rd), [fc] "v"(F), [A] "v"(A), [E] "v"(E), [t0] "v"(L.t0),
           [t1] "v"(L.t1), [t2] "v"(L.t2), [t3] "v"(L.t3), [ksh] "v"(L.ksh), [km] "v"(L.km), [three] "v"(L.three),
-          [mask] "v"(L.mask), [nlab] "v"(L.nlab), [hi] "v"(K.hi)
+          [nlab] "v"(L.nlab), [hi] "v"(K.hi)
         : "scc");
     if (NODES) nodes += (uint64_t)(W != 0u);             // the inert frame is no node
     uint32_t l1, r1, c1, w1, dA;
@@ -370,18 +372,18 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
         "v_cndmask_b32_e64 %[dA], %[dA], %[S], %[push]\n\t"
         "v_cndmask_b32_e64 %[r1], %[rd], %[ty], %[up]\n\t"
         "v_cndmask_b32_e64 %[lo], %[l1], %[nld], %[down]\n\t"
-        "v_cndmask_b32_e64 %[c1], %[col], %[tz], %[up]\n\t"
+        "v_cndmask_b32_e64 %[c1], %[fc], %[tz], %[up]\n\t"
         "v_cndmask_b32_e64 %[ro], %[r1], %[nrd], %[down]\n\t"
         "v_cndmask_b32_e64 %[w1], %[rest], %[tw], %[up]\n\t"
-        "v_cndmask_b32_e64 %[co], %[c1], %[ncol], %[down]\n\t"
+        "v_cndmask_b32_e64 %[co], %[c1], %[nfc], %[down]\n\t"
         "v_add_u32 %[Ao], %[A], %[dA]\n\t"
         "v_cndmask_b32_e64 %[Wo], %[w1], %[Wc], %[down]"
-        : [lo] "=&v"(w.ld), [ro] "=&v"(w.rd), [co] "=&v"(w.col), [Wo] "=&v"(w.W), [Ao] "=&v"(w.A),
+        : [lo] "=&v"(w.ld), [ro] "=&v"(w.rd), [co] "=&v"(w.fc), [Wo] "=&v"(w.W), [Ao] "=&v"(w.A),
           [l1] "=&v"(l1), [r1] "=&v"(r1), [c1] "=&v"(c1), [w1] "=&v"(w1), [dA] "=&v"(dA)
         : [up] "s"(up), [down] "s"(down), [push] "s"(push), [pop] "s"(pop), [tx] "v"(top.x), [ty] "v"(top.y),
-          [tz] "v"(top.z), [tw] "v"(top.w), [ld] "v"(ld), [rd] "v"(rd), [col] "v"(C), [rest] "v"(rest),
-          [nld] "v"(nld), [nrd] "v"(nrd), [ncol] "v"(ncol), [Wc] "v"(Wc), [A] "v"(A), [S] "v"(K.S), [nS] "v"(K.nS));
-    if (__builtin_amdgcn_inverse_ballot_w64(push)) st.write(A + st.S, make_uint4(ld, rd, C, rest));
+          [tz] "v"(top.z), [tw] "v"(top.w), [ld] "v"(ld), [rd] "v"(rd), [fc] "v"(F), [rest] "v"(rest),
+          [nld] "v"(nld), [nrd] "v"(nrd), [nfc] "v"(nfc), [Wc] "v"(Wc), [A] "v"(A), [S] "v"(K.S), [nS] "v"(K.nS));
+    if (__builtin_amdgcn_inverse_ballot_w64(push)) st.write(A + st.S, make_uint4(ld, rd, F, rest));
 }
 #else
 // host pass of the kernels only (kernels never run on the host)

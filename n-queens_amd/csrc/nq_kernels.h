@@ -7,7 +7,7 @@ namespace nqk {
 
 constexpr int kStackCap = 16;  // private-memory DFS stack entries (>= T-1, T <= 16)
 constexpr uint32_t kSchedU4 = 2;   // uint4 words of a record's schedule (nqt::sched_words)
-// A work item, 3 x uint4: {ld, rd, col, frame word}, {record, T | wshift << 8 |
+// A work item, 3 x uint4: {ld, rd, free columns, frame word}, {record, T | wshift << 8 |
 // N << 16, terminal mask, 0}, and the record's parameter bytes (schedule words 0-3).
 constexpr uint32_t kItemU4 = 3;
 

@@ -44,7 +44,7 @@ __device__ __forceinline__ void run_root(const Sched &S, uint4 root, Stack &st, 
     if (S.T == 0u) { cnt = (root.w & S.tm) ? 1u : 0u; return; }
     const LaneK L = lane_consts(S);
     const PairK K = pair_consts(st.S);
-    Walk w{root.x, root.y, root.z, start_word(S, root.w, 0u), st.empty()};
+    Walk w{root.x, root.y, free_cols(S, root.z), start_word(S, root.w, 0u), st.empty()};
     do {
         uint32_t c32 = 0;
 #pragma unroll
@@ -152,13 +152,15 @@ __global__ void k_expand(const uint4 *__restrict__ root, const uint4 *__restrict
             if (WRITE && k == 0u) raw[c] = (r.w & S.tm) ? 1u : 0u;
         } else if (S.T < 2u || split == 0u || split > S.T - 2u) {
             // the root itself is the single item (T = 1 roots are resolved by the DFS kernel)
-            if (k == 0u) put(r.x, r.y, r.z, start_word(S, r.w, 0u));
+            if (k == 0u) put(r.x, r.y, free_cols(S, r.z), start_word(S, r.w, 0u));
         } else {
             uint32_t av = r.w;
             for (uint32_t i = 0; i < k && av; i++) av &= av - 1u;
             if (av) {
                 const uint32_t upw = start_word(S, 0u, split);
-                auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) { put(a, b, cc, f | upw); };
+                auto emit = [&](uint32_t a, uint32_t b, uint32_t cc, uint32_t f) {
+                    put(a, b, free_cols(S, cc), f | upw);
+                };
                 // fan 1: the lane walks every candidate of its root
                 expand_subtree<NODES>(S, r.x, r.y, r.z, fan == 1u ? av : av & (0u - av), 0u, split, st, cnt, nn,
                                       emit);

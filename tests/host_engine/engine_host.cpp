@@ -32,7 +32,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     if (priv) {
         PrivStack st;
         st.init();
-        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
+        Walk w{ld, rd, free_cols(S, col), start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
@@ -40,7 +40,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
     } else {
         LdsStack st{m, 16u * block, 16u * (block + lane)};
         st.init();
-        Walk w{ld, rd, col, start_word(S, av, d0), st.empty()};
+        Walk w{ld, rd, free_cols(S, col), start_word(S, av, d0), st.empty()};
         do {
             done = walk_step<true>(L, w, st, cnt, nodes);
             if (done || cnt >= (1u << 31)) { total += cnt; cnt = 0; }
@@ -50,7 +50,7 @@ int64_t walk_item(const Sched &S, uint32_t ld, uint32_t rd, uint32_t col, uint32
         const Walk w0 = w;
         const uint64_t n0 = nodes;
         for (int k = 0; k < 3; k++) (void)walk_step<true>(L, w, st, cnt, nodes);
-        if (cnt || nodes != n0 || w.W != w0.W || w.A != w0.A || w.col != w0.col) return -2;
+        if (cnt || nodes != n0 || w.W != w0.W || w.A != w0.A || w.fc != w0.fc) return -2;
     }
     return (int64_t)total;
 }
