@@ -233,8 +233,6 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
     const uint32_t nrd = bop3<kOrAnd>((w.rd | bit) >> s, q >> L.ksh, L.km);
     const uint32_t nfc = F & ~bit;                       // the child's free columns
     uint32_t nf = bop3<kAndNotOr>(nfc, nld, nrd);
-    // nonzero when the frame has no prune
-    const uint32_t nla = q & L.nlab;
 #if defined(__HIP_DEVICE_COMPILE__)
     asm("" : "+v"(nf));   // the child's word below reuses nf instead of refolding it
     uint32_t x2;
@@ -244,8 +242,8 @@ NQ_HD inline bool walk_step(const LaneK &L, Walk &w, Stack &st, Count &cnt, uint
 #else
     const uint32_t x2 = x << 1;
 #endif
-    // the next row keeps a free square (or the frame has no prune)
-    const bool lok = (bop3<kAndNotOr>(nfc, x2, nrd >> 1) | nla) != 0u;
+    // the next row keeps a free square, or the frame has no prune (q's NLA bit)
+    const bool lok = bop3<kOrAnd>(bop3<kAndNotOr>(nfc, x2, nrd >> 1), q, L.nlab) != 0u;
     const uint32_t hiW = W & ~kAvailMask;                // r << 28
     const bool inner = W >= (1u << kRShift);             // children not on the terminal row
     const bool ok = lok & (nf != 0u);
@@ -338,8 +336,7 @@ __device__ inline void walk_step_paired(const LaneK &L, const PairK &K, Walk &w,
         "v_add_u32 %[a], %[a], %[a]\n\t"
         "v_lshrrev_b32 %[b], 1, %[nrd]\n\t"
         "v_bitop3_b32 %[a], %[nfc], %[a], %[b] bitop3:0x10\n\t"   // free squares of the next row
-        "v_and_b32 %[c], %[c], %[nlab]\n\t"
-        "v_or_b32 %[a], %[a], %[c]\n\t"                          // lookahead passes (or none here)
+        "v_bitop3_b32 %[a], %[a], %[c], %[nlab] bitop3:0xf8\n\t"  // | the no-prune flag: lookahead passes
         "v_bitop3_b32 %[b], %[nfc], %[nld], %[nrd] bitop3:0x10\n\t"  // nf
         "v_and_b32 %[d], %[W], %[hi]\n\t"                        // r << 28
         "v_add_co_u32_e64 %[e], %[m3], %[d], %[hi]\n\t"         // (r - 1) << 28; carries when r != 0 (inner)
